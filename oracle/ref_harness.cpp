@@ -1,0 +1,81 @@
+// ref_harness.cpp -- C-ABI shim over the reference's own CPU solver, compiled
+// from the reference sources where they lie (see oracle/Makefile, target ref).
+// Test infrastructure only: used to pin the oracle restatement and to write
+// golden fixtures; output goes to oracle/_ref/ and is never shipped.
+//
+// Wraps: PCG/PCG3D (src/solver/pcg.cpp, pcg_3d.cpp) with
+// UAAMGPreconditioner/UAAMGPreconditioner3D; Grid/Grid3D built with
+// lengths lx,ly,lz so dx = lx/nx exactly as mesh.h:21-22.
+#include "core/grid.h"
+#include "core/grid_3d.h"
+#include "solver/pcg.h"
+#include "solver/pcg_3d.h"
+#include "solver/preconditioner/uaamg_preconditioner.h"
+#include "solver/preconditioner/uaamg_preconditioner_3d.h"
+#include <memory>
+#include <vector>
+
+namespace {
+// Counts preconditioner applications: PCG applies once before its loop and
+// once per loop iteration that does not exit on the tolerance test.
+struct Count2D : Preconditioner {
+    UAAMGPreconditioner inner; int n = 0;
+    void apply(const Grid& g, const std::vector<double>& r, std::vector<double>& z) override { n++; inner.apply(g, r, z); }
+    std::string name() const override { return inner.name(); }
+};
+struct Count3D : Preconditioner3D {
+    UAAMGPreconditioner3D inner; int n = 0;
+    void apply(const Grid3D& g, const std::vector<double>& r, std::vector<double>& z) override { n++; inner.apply(g, r, z); }
+    std::string name() const override { return inner.name(); }
+};
+}
+
+extern "C" int ref_pcg(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                       const unsigned char* solid, const double* rhs, double* p,
+                       int max_iter, double tol, int* applies) {
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        size_t N = g.p.size();
+        for (size_t c = 0; c < N; c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+        std::vector<double> r(rhs, rhs + N);
+        auto pc = std::make_unique<Count2D>(); Count2D* raw = pc.get();
+        PCG s(std::move(pc));
+        s.solve(g, r, max_iter, tol);
+        for (size_t c = 0; c < N; c++) p[c] = g.p[c];
+        *applies = raw->n;
+        return 0;
+    }
+    Grid3D g(nx, ny, nz, lx, ly, lz);
+    size_t N = g.p.size();
+    for (size_t c = 0; c < N; c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+    std::vector<double> r(rhs, rhs + N);
+    auto pc = std::make_unique<Count3D>(); Count3D* raw = pc.get();
+    PCG3D s(std::move(pc));
+    s.solve(g, r, max_iter, tol);
+    for (size_t c = 0; c < N; c++) p[c] = g.p[c];
+    *applies = raw->n;
+    return 0;
+}
+
+extern "C" int ref_uaamg_apply(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                               const unsigned char* solid, const double* r, double* z) {
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        size_t N = g.p.size();
+        for (size_t c = 0; c < N; c++) g.solid[c] = solid[c] != 0;
+        std::vector<double> rv(r, r + N), zv(z, z + N);
+        UAAMGPreconditioner pc; pc.apply(g, rv, zv);
+        for (size_t c = 0; c < N; c++) z[c] = zv[c];
+        return 0;
+    }
+    Grid3D g(nx, ny, nz, lx, ly, lz);
+    size_t N = g.p.size();
+    for (size_t c = 0; c < N; c++) g.solid[c] = solid[c] != 0;
+    std::vector<double> rv(r, r + N), zv(z, z + N);
+    UAAMGPreconditioner3D pc; pc.apply(g, rv, zv);
+    for (size_t c = 0; c < N; c++) z[c] = zv[c];
+    return 0;
+}
+
+// dx exactly as the reference's Grid computes it, for building matching inputs.
+extern "C" double ref_spacing(double l, int n) { Grid g(n, 2, l, 1.0); return g.dx; }

@@ -1,0 +1,215 @@
+"""ctypes bindings to the CPU checker libraries (test infrastructure).
+
+oracle/liboracle.so  -- the C restatement of the reference solve (the oracle).
+oracle/_ref/libref.so -- the reference's own CPU solver compiled from its
+sources (present only where the reference tree was available at build time).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+NEUMANN, DIRICHLET = 0, 1
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("solid", C.c_void_p), ("wall", C.c_int * 6)]
+
+
+class OrcOpts(C.Structure):
+    _fields_ = [("nu_pre", C.c_int), ("nu_post", C.c_int), ("coarsest_side", C.c_int),
+                ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int), ("project_mean", C.c_int)]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("applies", C.c_int),
+                ("rnorm", C.c_double), ("r0norm", C.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(ORACLE_SO)
+        P = C.c_void_p
+        _lib.orc_default_opts.argtypes = [P]
+        _lib.orc_pcg.argtypes = [P, P, P, P, C.c_int, C.c_double, P]
+        _lib.orc_uaamg_apply.argtypes = [P, P, P, P]
+        _lib.orc_hierarchy_dims.argtypes = [P, P, P, P, C.c_int]
+        _lib.orc_level_data.argtypes = [P, P, C.c_int, P, P, P, P, P, P]
+        _lib.orc_matvec.argtypes = [P, P, P]
+        _lib.orc_amgpcg.argtypes = [C.c_int, P, P, C.c_int, C.c_double, P]
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_SO)
+        P = C.c_void_p
+        D, I = C.c_double, C.c_int
+        _ref.ref_pcg.argtypes = [I, I, I, I, D, D, D, P, P, P, I, D, P]
+        _ref.ref_uaamg_apply.argtypes = [I, I, I, I, D, D, D, P, P, P]
+        _ref.ref_amgpcg.argtypes = [I, P, P, I, D, P]
+        _ref.ref_spacing.argtypes = [D, I]
+        _ref.ref_spacing.restype = D
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Problem:
+    """A padded grid problem: dims n, lengths L (dx = L/n), solid mask, walls."""
+
+    def __init__(self, n, L=None, solid=None, wall=None):
+        self.dim = len(n)
+        self.n = tuple(int(v) for v in n)
+        self.L = tuple(float(v) for v in (L if L is not None else [1.0] * self.dim))
+        self.h = tuple(self.L[d] / self.n[d] for d in range(self.dim))
+        self.shape = tuple(v + 2 for v in self.n)          # (nx+2, ny+2[, nz+2]) i fastest
+        self.N = int(np.prod(self.shape))
+        if solid is None:
+            solid = np.zeros(self.N, np.uint8)
+        self.solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        self.wall = list(wall) if wall is not None else [NEUMANN] * 6
+
+    def grid(self):
+        g = OrcGrid()
+        g.dim = self.dim
+        g.nx, g.ny = self.n[0], self.n[1]
+        g.nz = self.n[2] if self.dim == 3 else 1
+        g.dx, g.dy = self.h[0], self.h[1]
+        g.dz = self.h[2] if self.dim == 3 else 1.0
+        g.solid = _p(self.solid)
+        for f in range(6):
+            g.wall[f] = self.wall[f]
+        self._keep = self.solid
+        return g
+
+    def view(self, a):
+        """Padded flat array -> array indexed [k][j][i] (or [j][i])."""
+        return a.reshape(self.shape[::-1])
+
+    def interior(self, a):
+        v = self.view(a)
+        if self.dim == 3:
+            return v[1:-1, 1:-1, 1:-1]
+        return v[1:-1, 1:-1]
+
+    def fluid_interior_mask(self):
+        m = np.zeros(self.N, bool)
+        self.interior(m)[...] = True
+        return m & (self.solid == 0)
+
+
+def default_opts(**kw):
+    o = OrcOpts()
+    lib().orc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def pcg(pb, rhs, max_iter, tol, p0=None, **opts):
+    p = np.zeros(pb.N) if p0 is None else np.array(p0, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    g = pb.grid()
+    o = default_opts(**opts)
+    res = OrcResult()
+    rc = lib().orc_pcg(C.byref(g), C.byref(o), _p(rhs), _p(p), max_iter, tol, C.byref(res))
+    assert rc == 0, rc
+    return p, res
+
+
+def uaamg_apply(pb, r, **opts):
+    z = np.zeros(pb.N)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    g = pb.grid()
+    o = default_opts(**opts)
+    rc = lib().orc_uaamg_apply(C.byref(g), C.byref(o), _p(r), _p(z))
+    assert rc == 0, rc
+    return z
+
+
+def matvec(pb, v):
+    Av = np.zeros(pb.N)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    g = pb.grid()
+    assert lib().orc_matvec(C.byref(g), _p(v), _p(Av)) == 0
+    return Av
+
+
+def hierarchy_dims(pb, **opts):
+    g = pb.grid()
+    o = default_opts(**opts)
+    nl = C.c_int(0)
+    dims = np.zeros(3 * 64, np.int32)
+    assert lib().orc_hierarchy_dims(C.byref(g), C.byref(o), C.byref(nl), _p(dims), 64) == 0
+    return [tuple(dims[3 * l:3 * l + pb.dim]) for l in range(nl.value)]
+
+
+def level_data(pb, level, **opts):
+    dims = hierarchy_dims(pb, **opts)[level]
+    N = int(np.prod([d + 2 for d in dims]))
+    out = {k: np.zeros(N) for k in ("diag", "cx", "cy", "cz", "excess")}
+    solid = np.zeros(N, np.uint8)
+    g = pb.grid()
+    o = default_opts(**opts)
+    rc = lib().orc_level_data(C.byref(g), C.byref(o), level, _p(solid), _p(out["diag"]), _p(out["cx"]),
+                              _p(out["cy"]), _p(out["cz"]), _p(out["excess"]))
+    assert rc == 0, rc
+    out["solid"] = solid
+    out["dims"] = dims
+    return out
+
+
+def amgpcg(N, b, max_iter, tol):
+    x = np.zeros((N + 2) * (N + 2))
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    it = C.c_int(0)
+    assert lib().orc_amgpcg(N, _p(b), _p(x), max_iter, tol, C.byref(it)) == 0
+    return x, it.value
+
+
+# ---- the reference's own solver (oracle/_ref), same calling conventions ----
+
+def ref_pcg(pb, rhs, max_iter, tol, p0=None):
+    p = np.zeros(pb.N) if p0 is None else np.array(p0, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    ap = C.c_int(0)
+    nz = pb.n[2] if pb.dim == 3 else 1
+    lz = pb.L[2] if pb.dim == 3 else 1.0
+    ref().ref_pcg(pb.dim, pb.n[0], pb.n[1], nz, pb.L[0], pb.L[1], lz, _p(pb.solid), _p(rhs), _p(p),
+                  max_iter, tol, C.byref(ap))
+    return p, ap.value
+
+
+def ref_uaamg_apply(pb, r):
+    z = np.zeros(pb.N)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    nz = pb.n[2] if pb.dim == 3 else 1
+    lz = pb.L[2] if pb.dim == 3 else 1.0
+    ref().ref_uaamg_apply(pb.dim, pb.n[0], pb.n[1], nz, pb.L[0], pb.L[1], lz, _p(pb.solid), _p(r), _p(z))
+    return z
+
+
+def ref_amgpcg(N, b, max_iter, tol):
+    x = np.zeros((N + 2) * (N + 2))
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    it = C.c_int(0)
+    ref().ref_amgpcg(N, _p(b), _p(x), max_iter, tol, C.byref(it))
+    return x, it.value
