@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark: PCG solves/s and HBM GB/s at 256^3 (relres 1e-6), one MI355X.
+
+One step = one complete cold-start pressure solve of the 256^3 plume
+workload (8 seeded solid spheres + floor slab, Neumann walls, zero-mean
+divergence rhs), including the per-solve hierarchy set-up from the solid mask
+on device, in the plain FP32 form (FP32 CG vectors and levels, FP64
+reductions/scalars), stopping at relative residual 1e-6.  Inputs (mask, rhs)
+are resident in HBM before the timed region.
+
+Prints one JSON line on stdout; diagnostics go to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PCG solves/s & HBM GB/s at 256^3 (relres 1e-6); served p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+KNAMES = ["matvec_dot", "update", "down0", "up0", "pupdate", "coarse", "init", "other"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def per_cell_bytes(cls, sv, sl, dim):
+    """Algorithmic HBM bytes per interior cell for one launch of a level-0 kernel
+    class (each array read or written once; nbr = 1-byte neighbour mask)."""
+    rho = 1.0 / 8 if dim == 3 else 1.0 / 4
+    return {
+        "matvec_dot": 2 * sv + 1,          # read p, nbr; write Ap
+        "update": 6 * sv + 1,              # read x, p, r, Ap, nbr; write x, r
+        "down0": sv + 1 + rho * sl,        # read r, nbr; write coarse b
+        "up0": 2 * sv + 1 + rho * sl,      # read r, nbr, coarse x; write z
+        "pupdate": 3 * sv + 1,             # read z, p, nbr; write p
+    }.get(cls)
+
+
+def cpu_baseline(pb, k_iters, budget_s=30.0):
+    """Time the reference CPU solver (oracle/_ref, compiled from the reference
+    sources) or, when absent, the oracle restatement, single thread, on a
+    bounded sample of the same workload: solves with max_iter = 1 and 2 give
+    t_iter = t2 - t1 and t_setup = t1 - t_iter; a full solve is extrapolated as
+    t_setup + k * t_iter with k the GPU solve's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    kind = "reference" if ob.ref_available() else "port"
+    opb = ob.Problem(pb.n, pb.L, pb.solid)
+    tol = 1e-30   # never met: the sample runs exactly max_iter iterations
+
+    def run(mi):
+        t = time.perf_counter()
+        if kind == "reference":
+            ob.ref_pcg(opb, pb.rhs, mi, tol)
+        else:
+            ob.pcg(opb, pb.rhs, mi, tol)
+        return time.perf_counter() - t
+
+    t1 = run(1)
+    t2 = run(2)
+    t_iter = max(t2 - t1, 1e-9)
+    t_setup = max(t1 - t_iter, 0.0)
+    t_solve = t_setup + k_iters * t_iter
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": 1.0 / t_solve,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"{'oracle/_ref (reference PCG3D+UAAMGPreconditioner3D, g++ -O2)' if kind == 'reference' else 'oracle restatement (C, gcc -O2)'}"
+                   f" FP64 1 thread on {pb.name}: max_iter=1 took {t1:.2f}s, max_iter=2 took {t2:.2f}s;"
+                   f" solve extrapolated as setup {t_setup:.2f}s + {k_iters} x {t_iter:.2f}s/iter;"
+                   f" host {cpu}, nproc {os.cpu_count()}"),
+        "s_per_iter": t_iter,
+    }
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), if a
+    measurement for this workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="grid side (the metric is quoted at 256)")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "independent solves shard per GPU with no communication, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import uaamg_amd as U
+    from uaamg_amd import workloads as W
+
+    prec = {"fp32": U.FP32, "fp64": U.FP64, "bf16": U.MIXED_BF16, "fp16": U.MIXED_FP16}[args.precision]
+    t0 = time.perf_counter()
+    pb = W.plume3d(seed=0, n=args.n)
+    log(f"[bench] generated {pb.name} in {time.perf_counter() - t0:.1f}s, solid fraction {pb.solid.mean():.3f}")
+
+    dev = torch.device("cuda:0")
+    rhs_d = torch.from_numpy(pb.rhs).to(dev)
+    solid_d = torch.from_numpy(pb.solid).to(dev)
+    p_d = torch.zeros(pb.N, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    ctx = U.Context(precision=prec, tol_mode=U.TOL_REL, max_iter=1000, tol=1e-6, check_every=4)
+    grid = U.capi.make_grid(pb.n, pb.h, np.zeros(1, np.uint8), pb.wall)
+    grid.solid = ctypes.c_void_p(solid_d.data_ptr())
+    ctx.n = pb.n; ctx.dim = pb.dim; ctx.N = pb.N
+
+    def step():
+        rc = ctx.lib.L.uaamg_set_grid_device(ctx.h, ctypes.byref(grid))
+        if rc != 0:
+            raise RuntimeError(f"set_grid_device: {U.status_name(rc)}")
+        p_d.zero_()
+        return ctx.solve_device(rhs_d.data_ptr(), p_d.data_ptr(), 0, 1000, 1e-6)
+
+    for _ in range(args.warmup):
+        st = step()
+    torch.cuda.synchronize()
+
+    # timed region: K complete solves
+    ctx.set_opts(timing=0)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    iters = []
+    alg = 0.0
+    for _ in range(args.steps):
+        st = step()
+        iters.append(st.iters)
+        alg += st.alg_bytes
+        if st.status != 0:
+            raise RuntimeError(f"solve did not converge: {U.status_name(st.status)} after {st.iters} iterations")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    ms_per_step = 1e3 * t_total / args.steps
+    solves_per_s = args.steps / t_total
+    hbm_gbs = alg / t_total / 1e9
+
+    # per-kernel device time with HIP events on the library's stream, over the
+    # same K solves repeated with event timing on
+    mask = 0xFF if args.profile_all else (1 << U.capi.K_MATVEC) | (1 << U.capi.K_UPDATE) | (1 << U.capi.K_DOWN0) | (1 << U.capi.K_UP0) | (1 << U.capi.K_PUPDATE)
+    ctx.set_opts(timing=mask)
+    ms_k = np.zeros(8); n_k = np.zeros(8)
+    for _ in range(args.steps):
+        st = step()
+        ms_k += np.array(st.ms_kernel[:]); n_k += np.array(st.n_kernel[:])
+    ctx.set_opts(timing=0)
+    torch.cuda.synchronize()
+    for c in range(8):
+        if n_k[c]:
+            log(f"[bench] {KNAMES[c]:>10s}: {ms_k[c] / n_k[c] * 1e3:8.1f} us/launch x {int(n_k[c] / args.steps)} per solve")
+    sv = 8 if prec == U.FP64 else 4
+    sl = {U.FP64: 8, U.FP32: 4}.get(prec, 2)
+    cand = [c for c in range(5) if n_k[c] > 0]
+    dom = max(cand, key=lambda c: ms_k[c])
+    dname = KNAMES[dom]
+    avg_s = ms_k[dom] / n_k[dom] / 1e3
+    bytes_launch = per_cell_bytes(dname, sv, sl, pb.dim) * pb.cells
+    achieved = bytes_launch / avg_s / 1e9
+    traffic = load_traffic(f"{pb.name}:{args.precision}:{dname}")
+
+    k_med = int(np.median(iters))
+    out = {
+        "metric": METRIC,
+        "value": round(solves_per_s, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "fp32", "fp64": "fp64", "bf16": "fp32+bf16", "fp16": "fp32+fp16"}[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": f"{pb.name}: 256^3 MAC pressure solve, dx=1/256, Neumann walls, floor slab y<0.1 + 8 seeded spheres, "
+                        "buoyant-plume zero-mean divergence rhs (seed 0); cold start, set-up included, relres 1e-6",
+            "grid": list(pb.n),
+            "relres": 1e-6,
+            "precision": {"fp32": "FP32 vectors+levels, FP64 reductions", "fp64": "FP64",
+                          "bf16": "FP32 CG, BF16 levels", "fp16": "FP32 CG, FP16 levels"}[args.precision],
+            "iters": k_med,
+            "hbm_gbs_solve": round(hbm_gbs, 1),
+            "alg_bytes_per_solve": alg / args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(pb, k_med)
+        except Exception as e:   # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

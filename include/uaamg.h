@@ -1,0 +1,158 @@
+/*
+ * uaamg.h -- C-ABI of the MI355X (gfx950) UAAMG-preconditioned CG pressure
+ * solve.  Plain pointers and sizes only; every entry point returns a status
+ * code (UAAMG_OK == 0).  Thread safety: one context per host thread.
+ *
+ * Layout contract (reference include/core/mesh.h:32-35, mesh_3d.h:78-82):
+ * cell arrays are ghost-padded, i fastest,
+ *     idx = i + j*(nx+2) [+ k*(nx+2)*(ny+2)],   interior i,j,k = 1..n,
+ * spacing h = L/n (mesh.h:21-22).  For 2D pass dim = 2 and n[2] = 1.
+ *
+ * Which reference interface each entry point replaces is stated above it
+ * (paths relative to the reference root).
+ */
+#ifndef UAAMG_H
+#define UAAMG_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+    UAAMG_OK = 0,
+    UAAMG_ERR_ARG = 1,         /* bad argument (null pointer, bad dims, no grid set) */
+    UAAMG_ERR_HIP = 2,         /* HIP runtime error (message in uaamg_last_error) */
+    UAAMG_ERR_OOM = 3,         /* device allocation failed */
+    UAAMG_ERR_NODEVICE = 4,    /* no usable gfx950 device */
+    UAAMG_ERR_ADMISSION = 5,   /* request exceeds the context's device-memory budget */
+    UAAMG_NOT_CONVERGED = 10,  /* max_iter reached (reference: silent loop exit) */
+    UAAMG_BREAKDOWN = 11,      /* p.Ap < 1e-15 (reference pcg.cpp:100: silent return) */
+    UAAMG_NONFINITE = 12       /* NaN/Inf in a CG scalar */
+};
+
+/* ---- precision of the device path ---- */
+enum {
+    UAAMG_FP64 = 0,            /* everything FP64: parity mode */
+    UAAMG_FP32 = 1,            /* plain form: FP32 vectors and levels, FP64 reductions/scalars */
+    UAAMG_MIXED_BF16 = 2,      /* FP32 CG; coarse levels stored BF16, FP32 arithmetic */
+    UAAMG_MIXED_FP16 = 3       /* FP32 CG; coarse levels stored FP16 (h^2-normalised), FP32 arithmetic */
+};
+
+enum { UAAMG_WALL_NEUMANN = 0, UAAMG_WALL_DIRICHLET = 1 };
+enum { UAAMG_TOL_ABS = 0, UAAMG_TOL_REL = 1 };
+
+typedef struct {
+    int dim;                   /* 2 or 3 */
+    int n[3];                  /* interior cells per axis; n[2] = 1 for 2D */
+    double h[3];               /* spacing per axis; h[2] ignored for 2D */
+    const uint8_t* solid;      /* padded mask, 1 = solid (host pointer, or device pointer for *_device calls) */
+    int wall[6];               /* -x,+x,-y,+y,-z,+z; reference walls are all Neumann */
+} uaamg_grid;
+
+typedef struct {
+    int precision;             /* UAAMG_FP64 | UAAMG_FP32 | UAAMG_MIXED_* */
+    int nu_pre, nu_post;       /* RBGS sweeps before/after the coarse correction (reference 1,1) */
+    int coarsest_side;         /* hierarchy stops once any side <= this (reference 4) */
+    int coarsest_sweeps;       /* symmetric sweep pairs on the coarsest level (reference 10) */
+    int tol_mode;              /* UAAMG_TOL_ABS (pcg.cpp:112) | UAAMG_TOL_REL (cuda_pcg_3d_opt.cu:410) */
+    int max_iter;              /* default for uaamg_solve* when max_iter < 0 is passed */
+    double tol;                /* default tolerance when tol < 0 is passed */
+    int project_mean;          /* -1 auto (on iff all walls Neumann), 0 off, 1 on */
+    int fused;                 /* 1: fused LDS V-cycle kernels (needs nu_pre == nu_post == 1); 0: per-pass kernels */
+    int check_every;           /* CG iterations queued between host convergence checks (>=1) */
+    int use_graph;             /* 1: replay queued iterations from a captured hipGraph */
+    int timing;                /* bitmask of UAAMG_K_* kernel classes timed with HIP events (0 off, 0xff all) */
+    int flexible;              /* 0: beta = (r,z)/rsold (pcg_3d.cpp:241); 1: Polak-Ribiere (flexible CG) */
+} uaamg_opts;
+
+typedef struct {
+    int iters;                 /* CG loop iterations (matvecs) performed */
+    int status;                /* UAAMG_OK | UAAMG_NOT_CONVERGED | UAAMG_BREAKDOWN | UAAMG_NONFINITE */
+    int nlevels;
+    double rnorm;              /* sqrt(r.r) at exit */
+    double r0norm;             /* sqrt(r.r) of the initial (projected, negated) residual */
+    double ms_total;           /* wall time of the call (host clock) */
+    double ms_setup;           /* reserved */
+    double ms_kernel[8];       /* device time per kernel class timed (opts.timing bit set), see UAAMG_K_* */
+    int    n_kernel[8];        /* launches per timed class */
+    double alg_bytes;          /* canonical algorithmic HBM bytes of this solve */
+} uaamg_stats;
+
+/* kernel classes for uaamg_stats.ms_kernel */
+enum { UAAMG_K_MATVEC = 0, UAAMG_K_UPDATE = 1, UAAMG_K_DOWN0 = 2, UAAMG_K_UP0 = 3,
+       UAAMG_K_PUPDATE = 4, UAAMG_K_COARSE = 5, UAAMG_K_INIT = 6, UAAMG_K_OTHER = 7 };
+
+typedef struct uaamg_ctx uaamg_ctx;
+
+/* Defaults reproduce the reference: FP64, V(1,1), coarsest side 4, 10 coarsest
+ * pairs, absolute tolerance, mean projection auto. */
+int uaamg_default_opts(uaamg_opts* o);
+
+/* Replaces constructing PCG(UAAMGPreconditioner) / CudaPCG3D
+ * (src/solver/factory.cpp:23, include/solver/cuda/cuda_pcg_3d.h:39-81):
+ * one context owns the device hierarchy and CG buffers. */
+int uaamg_create(uaamg_ctx** out, const uaamg_opts* o);
+void uaamg_destroy(uaamg_ctx* ctx);
+const char* uaamg_last_error(const uaamg_ctx* ctx);
+int uaamg_set_opts(uaamg_ctx* ctx, const uaamg_opts* o);
+
+/* Uploads the mask and builds the hierarchy on device: coarse masks,
+ * Galerkin couplings (CudaUAAMGPreconditioner3DT::setupLevels,
+ * cuda_uaamg_preconditioner_3d.cu:324-376).  Re-uploads on every call, so a
+ * mask change without a resize is honoured (the reference wrapper's
+ * stale-mask case, cuda_pcg_solver.cu:45-49). g->solid is a host pointer. */
+int uaamg_set_grid(uaamg_ctx* ctx, const uaamg_grid* g);
+/* Same, g->solid is a device pointer. */
+int uaamg_set_grid_device(uaamg_ctx* ctx, const uaamg_grid* g);
+
+/* Replaces Solver::solve(Grid&, rhs, max_iter, tol) for key "pcg_uaamg"
+ * (include/solver/solver.h:29-30, src/solver/pcg.cpp:18-126,
+ * pcg_3d.cpp:18-125): solves lap(p) = rhs; p[fluid interior] += x where x is
+ * the PCG iterate from x0 = 0 (pcg.cpp:107).  rhs_host, p_host: padded FP64
+ * host arrays.  max_iter < 0 / tol < 0 take the context defaults. */
+int uaamg_solve(uaamg_ctx* ctx, const double* rhs_host, double* p_host,
+                int max_iter, double tol, uaamg_stats* st);
+
+/* Replaces CudaPCG3D::solve / solve_optimized (cuda_pcg_3d.h:42-45) on device
+ * pointers: rhs_dev, p_dev are padded FP64 device arrays.  warm_start = 0:
+ * reference semantics (p += x, x from 0).  warm_start = 1: p_dev holds x0,
+ * the solve starts from r = b - A x0 and p_dev receives the solution. */
+int uaamg_solve_device(uaamg_ctx* ctx, const double* rhs_dev, double* p_dev,
+                       int warm_start, int max_iter, double tol, uaamg_stats* st);
+
+/* One V-cycle z = M^-1 r with a fresh hierarchy on the current grid
+ * (Preconditioner::apply, include/solver/preconditioner/preconditioner.h:28-29;
+ * uaamg_preconditioner.cpp:177-199). Host FP64 padded arrays; interior of z written. */
+int uaamg_apply_precond(uaamg_ctx* ctx, const double* r_host, double* z_host);
+
+/* The PCG operator Av (pcg.cpp:35-61, pcg_3d.cpp:164-181) and p.Av over fluid
+ * cells, as the HIP matvec+dot kernel computes them. Host FP64 arrays. */
+int uaamg_matvec(uaamg_ctx* ctx, const double* v_host, double* Av_host, double* vAv);
+
+/* Level count and per-level dims (n per axis), as built by set_grid. */
+int uaamg_hierarchy(const uaamg_ctx* ctx, int* nlevels, int* dims, int max_levels);
+
+/* Download one level's mask and stencil channels (FP64 copies) for checking
+ * against restrictSolid/setupFineCoeffs/galerkinCoarsen
+ * (uaamg_preconditioner_3d.cpp:34-116).  Any pointer may be NULL. */
+int uaamg_level_data(uaamg_ctx* ctx, int level, uint8_t* solid, double* diag,
+                     double* cx, double* cy, double* cz, double* excess);
+
+/* Standalone Ax=b solve with the semantics of the reference's amgpcg
+ * (test/solver/test_amgpcg.cpp:29-70 with test/test_common.h): N x N
+ * node-centred Dirichlet Poisson, h = 1/(N+1), row-major b[i*(N+2)+j];
+ * GMG V-cycle with 2+2 RBGS sweeps, 20 at the coarsest; absolute tolerance.
+ * x receives the (N+2)^2 solution; *iters as the reference's out_iters.
+ * precision: UAAMG_FP64 or UAAMG_FP32. */
+int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, double tol,
+                 int* iters, int precision);
+
+/* Device memory (bytes) a context would hold for grid g under opts o. */
+size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,81 @@
+"""Seeded problem generators shared by the parity tests (small sizes)."""
+import numpy as np
+
+import oracle_bind as ob
+
+
+def coords(pb):
+    """Cell-centre coordinates (padded arrays, [k][j][i] or [j][i] order)."""
+    idx = np.meshgrid(*[np.arange(s) for s in pb.shape[::-1]], indexing="ij")
+    return [(idx[len(idx) - 1 - d] - 0.5) * pb.h[d] for d in range(pb.dim)]
+
+
+def balls(pb, centers, radii):
+    X = coords(pb)
+    m = np.zeros(pb.shape[::-1], bool)
+    for c, r in zip(centers, radii):
+        d2 = sum((X[q] - c[q]) ** 2 for q in range(pb.dim))
+        m |= d2 < r * r
+    return m.reshape(-1).astype(np.uint8)
+
+
+def random_solids(pb, rng, frac=0.15, nobj=6):
+    X = coords(pb)
+    m = np.zeros(pb.shape[::-1], bool)
+    for _ in range(nobj):
+        if rng.random() < 0.5:
+            c = [rng.uniform(0, pb.L[q]) for q in range(pb.dim)]
+            r = rng.uniform(0.05, 0.2) * min(pb.L)
+            m |= sum((X[q] - c[q]) ** 2 for q in range(pb.dim)) < r * r
+        else:
+            lo = [rng.uniform(0, pb.L[q]) for q in range(pb.dim)]
+            ext = [rng.uniform(0.05, 0.3) * pb.L[q] for q in range(pb.dim)]
+            box = np.ones_like(m)
+            for q in range(pb.dim):
+                box &= (X[q] >= lo[q]) & (X[q] <= lo[q] + ext[q])
+            m |= box
+        if m.mean() > frac:
+            break
+    return m.reshape(-1).astype(np.uint8)
+
+
+def gaussian_rhs(pb):
+    """exp(-0.01 |x - c|^2) in index units (test_cuda_uaamg.cu:70-75)."""
+    idx = np.meshgrid(*[np.arange(s) for s in pb.shape[::-1]], indexing="ij")
+    d2 = 0
+    for d in range(pb.dim):
+        ii = idx[len(idx) - 1 - d]
+        d2 = d2 + (ii - pb.n[d] / 2.0) ** 2
+    out = np.zeros(pb.N)
+    pb.interior(out)[...] = pb.interior(np.exp(-0.01 * d2).reshape(-1))
+    return out
+
+
+def random_rhs(pb, rng):
+    out = np.zeros(pb.N)
+    pb.interior(out)[...] = rng.standard_normal(pb.interior(out).shape)
+    return out
+
+
+def parity_cases():
+    """(name, Problem) pairs: the reference's own test geometries plus odd,
+    anisotropic and solid-heavy grids."""
+    rng = np.random.default_rng(1234)
+    out = []
+    out.append(("2d_32x16", ob.Problem((32, 16))))
+    pb = ob.Problem((64, 16), (4.0, 1.0))
+    pb.solid = balls(pb, [(1.0, 0.5)], [0.1])
+    out.append(("2d_64x16_cylinder", pb))
+    out.append(("2d_33x17_odd", ob.Problem((33, 17), (1.0, 0.7))))
+    pb = ob.Problem((48, 40), (1.0, 40.0 / 30.0))
+    pb.solid = random_solids(pb, rng)
+    out.append(("2d_48x40_aniso_solids", pb))
+    out.append(("3d_16x8x4", ob.Problem((16, 8, 4))))
+    pb = ob.Problem((16, 16, 16))
+    pb.solid = balls(pb, [(0.5, 0.5, 0.5)], [0.25])
+    out.append(("3d_16_sphere", pb))
+    out.append(("3d_17x9x5_odd", ob.Problem((17, 9, 5), (1.0, 0.9, 0.8))))
+    pb = ob.Problem((24, 20, 18), (1.0, 1.1, 0.7))
+    pb.solid = random_solids(pb, rng)
+    out.append(("3d_24x20x18_aniso_solids", pb))
+    return out

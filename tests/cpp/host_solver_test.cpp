@@ -1,0 +1,129 @@
+// Drives the C++ host mirror (include/uaamg_solver.hpp) the way the
+// reference's application drives its Solver plugins (Factory::create(key),
+// solver->solve(grid, rhs, max_iter, tol), g.p accumulated in place), and
+// writes inputs and results as raw float64 files for the Python parity test.
+//
+//   host_solver_test keys          key parsing / factory errors (no GPU)
+//   host_solver_test solve OUTDIR  GPU solves through the Solver interface
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "uaamg_solver.hpp"
+
+using namespace uaamg_host;
+
+static int fails = 0;
+#define CHECK(c) do { if (!(c)) { std::fprintf(stderr, "CHECK failed: %s (line %d)\n", #c, __LINE__); ++fails; } } while (0)
+
+static void write_f64(const std::string& path, const std::vector<double>& v) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); ++fails; return; }
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+    std::fclose(f);
+}
+
+static void write_u8(const std::string& path, const std::vector<bool>& v) {
+    std::vector<unsigned char> b(v.begin(), v.end());
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); ++fails; return; }
+    std::fwrite(b.data(), 1, b.size(), f);
+    std::fclose(f);
+}
+
+static int run_keys() {
+    const char* keys[] = {"jacobi", "rbgs", "cg", "pcg", "pcg_gmg", "pcg_amg", "pcg_uaamg"};
+    for (const char* k : keys) CHECK(std::string(solver_key(parse_solver_kind(k))) == k);
+    CHECK(parse_solver_kind("pcg-uaamg") == SolverKind::Unknown);
+    bool threw = false;
+    try { Factory::create("jacobi"); } catch (const std::invalid_argument&) { threw = true; }
+    CHECK(threw);
+    threw = false;
+    try { Factory3D::create("pcg_uaamg", Backend::CPU); } catch (const std::invalid_argument&) { threw = true; }
+    CHECK(threw);
+    std::printf("keys %s\n", fails ? "FAILED" : "ok");
+    return fails ? 1 : 0;
+}
+
+static int run_solve(const std::string& out) {
+    // 3D: 24x16x12 on 1.5 x 1.0 x 0.75 with a solid ball, Gaussian source.
+    {
+        Grid3D g(24, 16, 12, 1.5, 1.0, 0.75);
+        std::vector<double> rhs(g.p_size(), 0.0);
+        for (int k = 1; k <= g.nz; ++k)
+            for (int j = 1; j <= g.ny; ++j)
+                for (int i = 1; i <= g.nx; ++i) {
+                    double dx = i - 16.0, dy = j - 8.0, dz = k - 6.0;
+                    if (dx * dx + dy * dy + dz * dz < 9.0) g.set_solid(i, j, k);
+                    double ex = i - 6.0, ey = j - 9.0, ez = k - 5.0;
+                    rhs[g.ip(i, j, k)] = 1e3 * std::exp(-0.05 * (ex * ex + ey * ey + ez * ez));
+                }
+        uaamg_opts o;
+        uaamg_default_opts(&o);
+        o.tol_mode = UAAMG_TOL_REL;
+        auto s = Factory3D::create("pcg_uaamg", Backend::HIP, &o);
+        CHECK(s->name() == "PCG3D(HIP-UAAMG3D)");
+        s->solve(g, rhs, 300, 1e-10);
+        auto* hs = dynamic_cast<HipPCGSolver3D*>(s.get());
+        CHECK(hs && hs->last_status() == UAAMG_OK);
+        std::printf("3d iters %d status %d\n", hs->last_stats().iters, hs->last_status());
+        write_f64(out + "/g3_rhs.f64", rhs);
+        write_u8(out + "/g3_solid.u8", g.solid);
+        write_f64(out + "/g3_p1.f64", g.p);
+        // second solve of the same system: p accumulates (p += x)
+        s->solve(g, rhs, 300, 1e-10);
+        write_f64(out + "/g3_p2.f64", g.p);
+    }
+    // 2D: 40x24 on 2.0 x 1.2 with a solid box.
+    {
+        Grid g(40, 24, 2.0, 1.2);
+        std::vector<double> rhs(g.p_size(), 0.0);
+        for (int j = 1; j <= g.ny; ++j)
+            for (int i = 1; i <= g.nx; ++i) {
+                if (i >= 20 && i < 26 && j >= 8 && j < 15) g.set_solid(i, j);
+                rhs[g.ip(i, j)] = 1e3 * std::sin(0.3 * i) * std::cos(0.2 * j);
+            }
+        uaamg_opts o;
+        uaamg_default_opts(&o);
+        o.tol_mode = UAAMG_TOL_REL;
+        auto s = Factory::create("pcg_uaamg", Backend::HIP, &o);
+        s->solve(g, rhs, 300, 1e-10);
+        auto* hs = dynamic_cast<HipPCGSolver*>(s.get());
+        CHECK(hs && hs->last_status() == UAAMG_OK);
+        std::printf("2d iters %d status %d\n", hs->last_stats().iters, hs->last_status());
+        write_f64(out + "/g2_rhs.f64", rhs);
+        write_u8(out + "/g2_solid.u8", g.solid);
+        write_f64(out + "/g2_p1.f64", g.p);
+    }
+    // standalone amgpcg, N = 64, b = sin(pi x) sin(pi y) (test_amgpcg.cpp:29-70)
+    {
+        amg::Grid b(64);
+        for (int i = 1; i <= b.N; ++i)
+            for (int j = 1; j <= b.N; ++j)
+                b(i, j) = std::sin(M_PI * i * b.h) * std::sin(M_PI * j * b.h);
+        int iters = 0;
+        amg::Grid x = amg::amgpcg_solve(b, 100, 1e-10, iters);
+        std::printf("amgpcg iters %d\n", iters);
+        write_f64(out + "/amg_b.f64", b.v);
+        write_f64(out + "/amg_x.f64", x.v);
+        std::vector<double> it(1, (double)iters);
+        write_f64(out + "/amg_iters.f64", it);
+    }
+    std::printf("solve %s\n", fails ? "FAILED" : "ok");
+    return fails ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 2 && std::strcmp(argv[1], "keys") == 0) return run_keys();
+    if (argc >= 3 && std::strcmp(argv[1], "solve") == 0) {
+        try {
+            return run_solve(argv[2]);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "exception: %s\n", e.what());
+            return 2;
+        }
+    }
+    std::fprintf(stderr, "usage: %s keys | solve OUTDIR\n", argv[0]);
+    return 64;
+}

@@ -1,0 +1,71 @@
+"""GPU: the C++ host mirror (include/uaamg_solver.hpp) driven like the
+reference application drives its Solver plugins, checked against the oracle.
+The program runs as a child process; it links the HIP library through the
+C-ABI only."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_TOOL = os.path.join(ROOT, "build", "host_solver_test")
+
+pytestmark = pytest.mark.gpu
+
+
+def _rd(d, name, dt=np.float64):
+    return np.fromfile(os.path.join(d, name), dtype=dt)
+
+
+@pytest.fixture(scope="module")
+def host_run(tmp_path_factory, gpu):
+    if not os.path.exists(HOST_TOOL):
+        pytest.fail("build/host_solver_test not built (run __graft_entry__.build())")
+    d = str(tmp_path_factory.mktemp("host"))
+    print(f"host mirror: {HOST_TOOL} solve {d}", flush=True)
+    r = subprocess.run([HOST_TOOL, "solve", d], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    return d, r.stdout
+
+
+def _fluid_err(pb, p, po):
+    m = pb.fluid_interior_mask()
+    a, b = p[m] - p[m].mean(), po[m] - po[m].mean()
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_host_mirror_3d_matches_oracle(host_run):
+    d, _ = host_run
+    pb = ob.Problem((24, 16, 12), (1.5, 1.0, 0.75), solid=_rd(d, "g3_solid.u8", np.uint8))
+    rhs = _rd(d, "g3_rhs.f64")
+    po, res = ob.pcg(pb, rhs, 300, 1e-10, tol_mode=1)
+    assert res.status == 0
+    p1 = _rd(d, "g3_p1.f64")
+    assert _fluid_err(pb, p1, po) <= 1e-9
+    # solid and ghost cells untouched (pcg.cpp:107 writes fluid interior only)
+    assert np.all(p1[~pb.fluid_interior_mask()] == 0.0)
+    # a second solve accumulates: p2 = p1 + x, x solving the same system
+    p2 = _rd(d, "g3_p2.f64")
+    assert _fluid_err(pb, p2, 2 * po) <= 1e-9
+
+
+def test_host_mirror_2d_matches_oracle(host_run):
+    d, _ = host_run
+    pb = ob.Problem((40, 24), (2.0, 1.2), solid=_rd(d, "g2_solid.u8", np.uint8))
+    rhs = _rd(d, "g2_rhs.f64")
+    po, res = ob.pcg(pb, rhs, 300, 1e-10, tol_mode=1)
+    assert res.status == 0
+    assert _fluid_err(pb, _rd(d, "g2_p1.f64"), po) <= 1e-9
+
+
+def test_host_mirror_amgpcg_matches_oracle(host_run):
+    d, _ = host_run
+    b = _rd(d, "amg_b.f64")
+    xo, it = ob.amgpcg(64, b, 100, 1e-10)
+    x = _rd(d, "amg_x.f64")
+    assert int(_rd(d, "amg_iters.f64")[0]) == it
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()

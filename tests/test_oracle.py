@@ -1,0 +1,272 @@
+"""CPU tests of the oracle (the C restatement of the reference solve).
+
+Pins, in order of strength:
+  1. the golden fixtures written from the reference's own CPU solver
+     (tests/golden/make_golden.py over oracle/_ref): bit-for-bit equality;
+  2. the compiled reference itself (oracle/_ref), when present: bit-for-bit
+     on fresh random cases;
+  3. mathematics independent of the reference's code: graph-Laplacian
+     brute force for the operator, A 1 = 0, dense R^T A P for the Galerkin
+     coarse operator, symmetry of the V-cycle, a dense least-squares solve
+     for PCG, and the reference's analytic sin*sin tests for amgpcg.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import cases
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def _problem(d):
+    n = tuple(int(v) for v in d["n"])
+    return ob.Problem(n, tuple(float(v) for v in d["L"]), d["solid"])
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_matches_golden(path):
+    d = np.load(path, allow_pickle=False)
+    kind = str(d["kind"])
+    if kind == "apply":
+        pb = _problem(d)
+        z = ob.uaamg_apply(pb, d["r"])
+        m = np.zeros(pb.N, bool); pb.interior(m)[...] = True
+        assert np.array_equal(z[m], d["z"][m])
+    elif kind == "pcg":
+        pb = _problem(d)
+        p, res = ob.pcg(pb, d["rhs"], int(d["max_iter"]), float(d["tol"]))
+        assert np.array_equal(p, d["p"])
+        assert res.applies == int(d["applies"])
+    else:
+        N = int(d["N"])
+        b = d["b"] if "b" in d.files else None
+        if b is None:
+            b = np.zeros((N + 2) ** 2)
+            h = 1.0 / (N + 1)
+            ii = np.arange(1, N + 1)
+            b.reshape(N + 2, N + 2)[1:-1, 1:-1] = 2 * np.pi ** 2 * np.outer(np.sin(np.pi * ii * h), np.sin(np.pi * ii * h))
+        x, it = ob.amgpcg(N, b, int(d["max_iter"]), float(d["tol"]))
+        assert it == int(d["iters"])
+        if "x" in d.files:
+            assert np.array_equal(x, d["x"])
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (no reference tree here)")
+@pytest.mark.parametrize("name,pb", cases.parity_cases(), ids=[c[0] for c in cases.parity_cases()])
+def test_oracle_matches_reference_build(name, pb):
+    rng = np.random.default_rng(99)
+    r = cases.random_rhs(pb, rng)
+    m = np.zeros(pb.N, bool); pb.interior(m)[...] = True
+    assert np.array_equal(ob.uaamg_apply(pb, r)[m], ob.ref_uaamg_apply(pb, r)[m])
+    p1, res = ob.pcg(pb, 1e3 * r, 300, 1e-3)
+    p2, applies = ob.ref_pcg(pb, 1e3 * r, 300, 1e-3)
+    assert np.array_equal(p1, p2) and res.applies == applies
+
+
+# ---------------------------------------------------------------- operator
+
+def graph_laplacian(pb, v):
+    """(A v)_c = sum over fluid face-neighbours n of (v_c - v_n)/h_axis^2 on
+    fluid cells (Neumann walls and solid faces carry no flux)."""
+    fl = pb.fluid_interior_mask().reshape(pb.shape[::-1])
+    V = v.reshape(pb.shape[::-1])
+    out = np.zeros_like(V)
+    for ax in range(pb.dim):
+        h2 = pb.h[pb.dim - 1 - ax] ** 2
+        for sgn in (1, -1):
+            nb = np.roll(V, -sgn, axis=ax)
+            fn = np.roll(fl, -sgn, axis=ax)
+            out += np.where(fl & fn, (V - nb) / h2, 0.0)
+    return np.where(fl, out, 0.0).reshape(-1)
+
+
+@pytest.mark.parametrize("name,pb", cases.parity_cases(), ids=[c[0] for c in cases.parity_cases()])
+def test_matvec_is_graph_laplacian(name, pb):
+    rng = np.random.default_rng(5)
+    v = cases.random_rhs(pb, rng)
+    Av = ob.matvec(pb, v)
+    ref = graph_laplacian(pb, v)
+    m = pb.fluid_interior_mask()
+    assert np.allclose(Av[m], ref[m], rtol=1e-12, atol=1e-9 * np.abs(ref).max())
+    ones = np.zeros(pb.N); ones[m] = 1.0
+    assert np.abs(ob.matvec(pb, ones)[m]).max() <= 1e-9 * max(1.0 / h ** 2 for h in pb.h)
+
+
+def test_matvec_known_answers():
+    # 1D-like profile on a 4x1 strip: constant -> 0; linear ramp -> interior 0, ends -+1/h^2
+    pb = ob.Problem((4, 2), (4.0, 2.0))
+    v = np.zeros(pb.N)
+    V = pb.view(v)
+    V[1:-1, 1:-1] = np.arange(1, 5)[None, :]
+    Av = pb.view(ob.matvec(pb, v))
+    assert np.allclose(Av[1, 1:-1], [-1.0, 0.0, 0.0, 1.0])
+
+
+# ---------------------------------------------------------------- hierarchy
+
+def dense_level0(pb):
+    """Coefficient form of level 0 as a dense matrix over fluid cells."""
+    idx = np.flatnonzero(pb.fluid_interior_mask())
+    pos = {c: q for q, c in enumerate(idx)}
+    n = len(idx)
+    A = np.zeros((n, n))
+    strides = [1, pb.shape[0]] + ([pb.shape[0] * pb.shape[1]] if pb.dim == 3 else [])
+    for q, c in enumerate(idx):
+        for ax in range(pb.dim):
+            for sgn in (1, -1):
+                cn = c + sgn * strides[ax]
+                if cn in pos:
+                    w = 1.0 / pb.h[ax] ** 2
+                    A[q, pos[cn]] -= w
+                    A[q, q] += w
+    return A, idx
+
+
+@pytest.mark.parametrize("n,L", [((8, 6), (1.0, 0.75)), ((9, 7), (1.0, 1.0)), ((8, 6, 6), (1.0, 0.75, 0.5)),
+                                 ((9, 5, 6), (1.0, 1.0, 1.0))])
+def test_galerkin_is_RtAP(n, L):
+    """Without solids the reference's coarse stencil is exactly R A P with P
+    the constant injection of fluid children (sinks at odd truncation)."""
+    pb = ob.Problem(n, L)
+    A, idx = dense_level0(pb)
+    dims = ob.hierarchy_dims(pb)
+    c = dims[1]
+    cshape = tuple(v + 2 for v in c)
+    cidx = []
+    P = np.zeros((len(idx), int(np.prod(c))))
+    pos = {cc: q for q, cc in enumerate(idx)}
+    grid = np.stack(np.meshgrid(*[np.arange(1, v + 1) for v in c[::-1]], indexing="ij"), -1).reshape(-1, pb.dim)
+    for q, cc in enumerate(grid):          # cc in (slow..fast) order
+        ic = cc[::-1]
+        cidx.append(int(sum(ic[a] * int(np.prod(cshape[:a])) for a in range(pb.dim))))
+        for off in np.ndindex(*([2] * pb.dim)):
+            f = [2 * ic[a] - 1 + off[a] for a in range(pb.dim)]
+            fid = int(sum(f[a] * int(np.prod(pb.shape[:a])) for a in range(pb.dim)))
+            if fid in pos:
+                P[pos[fid], q] = 1.0
+    Ac = P.T @ A @ P
+    lv = ob.level_data(pb, 1)
+    cidx = np.array(cidx)
+    diag = lv["diag"][cidx]
+    assert np.allclose(np.diag(Ac), diag, rtol=1e-13)
+    cstr = [1, cshape[0]] + ([cshape[0] * cshape[1]] if pb.dim == 3 else [])
+    where = {v: q for q, v in enumerate(cidx)}
+    for q, cc in enumerate(cidx):
+        for ax, ch in enumerate(["cx", "cy", "cz"][:pb.dim]):
+            nb = cc + cstr[ax]
+            if nb in where:
+                assert np.isclose(-Ac[q, where[nb]], lv[ch][cc], rtol=1e-13)
+
+
+def test_galerkin_closed_form_uniform():
+    """Interior coarse couplings on a solid-free uniform grid grow by
+    2^(d-1) per level (cuda_uaamg_preconditioner_3d.cu:358-363)."""
+    for n, d in (((32, 32), 2), ((16, 16, 16), 3)):
+        pb = ob.Problem(n)
+        for lvl in range(1, len(ob.hierarchy_dims(pb))):
+            lv = ob.level_data(pb, lvl)
+            dims = lv["dims"]
+            shape = tuple(v + 2 for v in dims)
+            cx = lv["cx"].reshape(shape[::-1])
+            inner = cx[(slice(2, -2),) * d] if d == 3 else cx[2:-2, 2:-3]
+            expect = (1.0 / pb.h[0] ** 2) * (2 ** (d - 1)) ** lvl
+            vals = inner[inner != 0]
+            assert np.allclose(vals, expect, rtol=1e-14)
+
+
+def test_coarse_solid_threshold():
+    """restrictSolid: coarse solid iff >= 2 of 4 (2D) / >= 4 of 8 (3D) children solid."""
+    pb = ob.Problem((4, 4))
+    s = np.zeros(pb.shape[::-1], np.uint8)
+    s[1, 1] = 1                      # 1 of 4 -> fluid
+    s[1, 3] = s[2, 4] = 1            # 2 of 4 -> solid
+    pb.solid = s.reshape(-1)
+    lv = ob.level_data(pb, 1, coarsest_side=2)
+    cs = lv["solid"].reshape(4, 4)
+    assert cs[1, 1] == 0 and cs[1, 2] == 1
+    pb3 = ob.Problem((4, 4, 4))
+    s3 = np.zeros(pb3.shape[::-1], np.uint8)
+    s3[1, 1, 1] = s3[1, 2, 1] = s3[2, 1, 1] = 1            # 3 of 8 -> fluid
+    s3[1, 1, 3] = s3[1, 2, 3] = s3[2, 1, 3] = s3[2, 2, 4] = 1  # 4 of 8 -> solid
+    pb3.solid = s3.reshape(-1)
+    cs3 = ob.level_data(pb3, 1, coarsest_side=2)["solid"].reshape(4, 4, 4)
+    assert cs3[1, 1, 1] == 0 and cs3[1, 1, 2] == 1
+
+
+# ---------------------------------------------------------------- V-cycle and PCG
+
+@pytest.mark.parametrize("name,pb", cases.parity_cases(), ids=[c[0] for c in cases.parity_cases()])
+def test_vcycle_symmetric(name, pb):
+    """Forward pre-smooth, reverse post-smooth, symmetric coarse operators:
+    <M r1, r2> = <r1, M r2>."""
+    rng = np.random.default_rng(8)
+    m = pb.fluid_interior_mask()
+    r1, r2 = cases.random_rhs(pb, rng), cases.random_rhs(pb, rng)
+    r1[~m] = 0; r2[~m] = 0
+    a = np.dot(ob.uaamg_apply(pb, r1)[m], r2[m])
+    b = np.dot(r1[m], ob.uaamg_apply(pb, r2)[m])
+    assert abs(a - b) <= 1e-11 * max(abs(a), abs(b))
+
+
+@pytest.mark.parametrize("n,L,solid", [((12, 8), (1.0, 1.0), False), ((10, 9), (1.0, 0.6), True),
+                                       ((6, 5, 4), (1.0, 1.0, 1.0), False), ((7, 6, 5), (1.0, 0.8, 0.9), True)])
+def test_pcg_matches_dense_solve(n, L, solid):
+    pb = ob.Problem(n, L)
+    if solid:
+        s = np.zeros(pb.shape[::-1], np.uint8)
+        s[(slice(2, 4),) * pb.dim] = 1
+        pb.solid = s.reshape(-1)
+    rng = np.random.default_rng(2)
+    rhs = 1e3 * cases.random_rhs(pb, rng)
+    m = pb.fluid_interior_mask()
+    rhs[m] -= rhs[m].mean()
+    p, res = ob.pcg(pb, rhs, 200, 1e-10, tol_mode=1)
+    assert res.status == 0
+    A, idx = dense_level0(pb)
+    x = np.linalg.lstsq(A, -rhs[idx], rcond=None)[0]   # lap p = rhs  <=>  (-lap) p = -rhs
+    a = p[idx] - p[idx].mean()
+    b = x - x.mean()
+    assert np.abs(a - b).max() <= 1e-7 * np.abs(b).max()
+
+
+def test_pcg_zero_rhs_returns_unchanged():
+    """rhs = 0: pAp = 0 < 1e-15 at the first iteration, p untouched (pcg.cpp:100)."""
+    pb = ob.Problem((8, 8))
+    p0 = np.full(pb.N, 3.0)
+    p, res = ob.pcg(pb, np.zeros(pb.N), 10, 1e-6, p0=p0)
+    assert res.status == 2 and res.iters == 1 and np.array_equal(p, p0)
+
+
+def test_amgpcg_analytic():
+    """test_amgpcg.cpp:75-92: sin*sin at N=32, error < 1e-2 in <= 30 iterations."""
+    N = 32
+    h = 1.0 / (N + 1)
+    ii = np.arange(1, N + 1)
+    S = np.outer(np.sin(np.pi * ii * h), np.sin(np.pi * ii * h))
+    b = np.zeros((N + 2, N + 2)); b[1:-1, 1:-1] = 2 * np.pi ** 2 * S
+    x, it = ob.amgpcg(N, b.reshape(-1), 30, 1e-8)
+    assert it <= 30
+    assert np.abs(x.reshape(N + 2, N + 2)[1:-1, 1:-1] - S).max() < 1e-2
+
+
+def test_dirichlet_extension_second_order():
+    """Dirichlet walls (extension): lap p = 2 pi^2 sin sin on the unit square
+    converges to -sin sin at second order."""
+    errs = []
+    for n in (16, 32, 64):
+        pb = ob.Problem((n, n), wall=[ob.DIRICHLET] * 6)
+        h = 1.0 / n
+        c = (np.arange(n + 2) - 0.5) * h
+        X, Y = np.meshgrid(c, c)
+        rhs = 2 * np.pi ** 2 * np.sin(np.pi * X) * np.sin(np.pi * Y)
+        rhs[0, :] = rhs[-1, :] = rhs[:, 0] = rhs[:, -1] = 0
+        # scaled by 1e4 so the absolute pAp < 1e-15 guard stays far away
+        p, res = ob.pcg(pb, 1e4 * rhs.reshape(-1), 200, 1e-10, tol_mode=1)
+        assert res.status == 0
+        exact = -np.sin(np.pi * X) * np.sin(np.pi * Y)
+        errs.append(np.abs(1e-4 * p.reshape(n + 2, n + 2)[1:-1, 1:-1] - exact[1:-1, 1:-1]).max())
+    assert errs[1] < errs[0] / 3.2 and errs[2] < errs[1] / 3.2

@@ -1,0 +1,220 @@
+"""ctypes view of the C-ABI in include/uaamg.h."""
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+FP64, FP32, MIXED_BF16, MIXED_FP16 = 0, 1, 2, 3
+TOL_ABS, TOL_REL = 0, 1
+NEUMANN, DIRICHLET = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NODEVICE, ERR_ADMISSION = 0, 1, 2, 3, 4, 5
+NOT_CONVERGED, BREAKDOWN, NONFINITE = 10, 11, 12
+_NAMES = {0: "ok", 1: "bad-argument", 2: "hip-error", 3: "out-of-memory", 4: "no-device",
+          5: "admission", 10: "not-converged", 11: "breakdown", 12: "nonfinite"}
+K_MATVEC, K_UPDATE, K_DOWN0, K_UP0, K_PUPDATE, K_COARSE, K_INIT, K_OTHER = range(8)
+
+
+def status_name(rc):
+    return _NAMES.get(rc, str(rc))
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("h", C.c_double * 3),
+                ("solid", C.c_void_p), ("wall", C.c_int * 6)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
+                ("coarsest_side", C.c_int), ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int),
+                ("max_iter", C.c_int), ("tol", C.c_double), ("project_mean", C.c_int),
+                ("fused", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
+                ("timing", C.c_int), ("flexible", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("nlevels", C.c_int),
+                ("rnorm", C.c_double), ("r0norm", C.c_double), ("ms_total", C.c_double),
+                ("ms_setup", C.c_double), ("ms_kernel", C.c_double * 8), ("n_kernel", C.c_int * 8),
+                ("alg_bytes", C.c_double)]
+
+
+EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",
+           "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_solve", "uaamg_solve_device",
+           "uaamg_apply_precond", "uaamg_matvec", "uaamg_hierarchy", "uaamg_level_data",
+           "uaamg_amgpcg", "uaamg_bytes_required"]
+
+
+def lib_path():
+    return os.path.join(HERE, "lib", "libuaamg_hip.so")
+
+
+class Lib:
+    def __init__(self, path=None):
+        path = path or lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+        self.path = path
+        self.L = L = C.CDLL(path)
+        P, I, D = C.c_void_p, C.c_int, C.c_double
+        L.uaamg_default_opts.argtypes = [P]
+        L.uaamg_create.argtypes = [C.POINTER(C.c_void_p), P]
+        L.uaamg_destroy.argtypes = [P]
+        L.uaamg_destroy.restype = None
+        L.uaamg_last_error.argtypes = [P]
+        L.uaamg_last_error.restype = C.c_char_p
+        L.uaamg_set_opts.argtypes = [P, P]
+        L.uaamg_set_grid.argtypes = [P, P]
+        L.uaamg_set_grid_device.argtypes = [P, P]
+        L.uaamg_solve.argtypes = [P, P, P, I, D, P]
+        L.uaamg_solve_device.argtypes = [P, P, P, I, I, D, P]
+        L.uaamg_apply_precond.argtypes = [P, P, P]
+        L.uaamg_matvec.argtypes = [P, P, P, P]
+        L.uaamg_hierarchy.argtypes = [P, P, P, I]
+        L.uaamg_level_data.argtypes = [P, I, P, P, P, P, P, P]
+        L.uaamg_amgpcg.argtypes = [I, P, P, I, D, P, I]
+        L.uaamg_bytes_required.argtypes = [P, P]
+        L.uaamg_bytes_required.restype = C.c_size_t
+
+    def default_opts(self, **kw):
+        o = Opts()
+        self.L.uaamg_default_opts(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def amgpcg(self, N, b, max_iter, tol, precision=FP64):
+        x = np.zeros((N + 2) * (N + 2))
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        it = C.c_int(0)
+        rc = self.L.uaamg_amgpcg(N, _p(b), _p(x), max_iter, tol, C.byref(it), precision)
+        if rc != OK:
+            raise RuntimeError(f"uaamg_amgpcg failed: {status_name(rc)}")
+        return x, it.value
+
+
+_LIB = None
+
+
+def load(path=None):
+    global _LIB
+    if _LIB is None or path is not None:
+        _LIB = Lib(path)
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_grid(n, h, solid, wall=None):
+    g = GridDesc()
+    g.dim = len(n)
+    for q in range(3):
+        g.n[q] = int(n[q]) if q < len(n) else 1
+        g.h[q] = float(h[q]) if q < len(h) else 1.0
+    g.solid = _p(solid)
+    w = wall if wall is not None else [NEUMANN] * 6
+    for f in range(6):
+        g.wall[f] = int(w[f])
+    return g
+
+
+class Context:
+    """One solver context (device hierarchy + CG buffers) on its own HIP stream."""
+
+    def __init__(self, lib=None, **opts):
+        self.lib = lib or load()
+        self.opts = self.lib.default_opts(**opts)
+        h = C.c_void_p()
+        rc = self.lib.L.uaamg_create(C.byref(h), C.byref(self.opts))
+        if rc != OK:
+            raise RuntimeError(f"uaamg_create failed: {status_name(rc)}")
+        self.h = h
+        self.n = None
+
+    h = None
+
+    def close(self):
+        if self.h:
+            self.lib.L.uaamg_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _err(self, what, rc):
+        msg = self.lib.L.uaamg_last_error(self.h).decode()
+        raise RuntimeError(f"{what}: {status_name(rc)}: {msg}")
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        rc = self.lib.L.uaamg_set_opts(self.h, C.byref(self.opts))
+        if rc != OK:
+            self._err("set_opts", rc)
+
+    def set_grid(self, n, h, solid, wall=None):
+        self.n = tuple(int(v) for v in n)
+        self.dim = len(n)
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        self.N = int(np.prod([v + 2 for v in n]))
+        assert self._solid.size == self.N, (self._solid.size, self.N)
+        g = make_grid(n, h, self._solid, wall)
+        rc = self.lib.L.uaamg_set_grid(self.h, C.byref(g))
+        if rc != OK:
+            self._err("set_grid", rc)
+
+    def solve(self, rhs, max_iter=-1, tol=-1.0, p0=None):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        p = np.zeros(self.N) if p0 is None else np.array(p0, dtype=np.float64)
+        st = Stats()
+        rc = self.lib.L.uaamg_solve(self.h, _p(rhs), _p(p), int(max_iter), float(tol), C.byref(st))
+        if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
+            self._err("solve", rc)
+        return p, st
+
+    def solve_device(self, rhs_ptr, p_ptr, warm=0, max_iter=-1, tol=-1.0):
+        st = Stats()
+        rc = self.lib.L.uaamg_solve_device(self.h, C.c_void_p(rhs_ptr), C.c_void_p(p_ptr), int(warm),
+                                           int(max_iter), float(tol), C.byref(st))
+        if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
+            self._err("solve_device", rc)
+        return st
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.N)
+        rc = self.lib.L.uaamg_apply_precond(self.h, _p(r), _p(z))
+        if rc != OK:
+            self._err("apply", rc)
+        return z
+
+    def matvec(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        Av = np.zeros(self.N)
+        d = C.c_double(0)
+        rc = self.lib.L.uaamg_matvec(self.h, _p(v), _p(Av), C.byref(d))
+        if rc != OK:
+            self._err("matvec", rc)
+        return Av, d.value
+
+    def hierarchy(self):
+        nl = C.c_int(0)
+        dims = np.zeros(3 * 64, np.int32)
+        rc = self.lib.L.uaamg_hierarchy(self.h, C.byref(nl), _p(dims), 64)
+        if rc != OK:
+            self._err("hierarchy", rc)
+        return [tuple(dims[3 * l:3 * l + self.dim]) for l in range(nl.value)]
+
+    def level_data(self, level):
+        dims = self.hierarchy()[level]
+        N = int(np.prod([d + 2 for d in dims]))
+        out = {k: np.zeros(N) for k in ("diag", "cx", "cy", "cz", "excess")}
+        solid = np.zeros(N, np.uint8)
+        rc = self.lib.L.uaamg_level_data(self.h, level, _p(solid), _p(out["diag"]), _p(out["cx"]),
+                                         _p(out["cy"]), _p(out["cz"]), _p(out["excess"]))
+        if rc != OK:
+            self._err("level_data", rc)
+        out["solid"] = solid
+        out["dims"] = dims
+        return out

@@ -1,0 +1,599 @@
+// engine.hpp -- host-side orchestration of one grid's hierarchy and PCG on a
+// HIP stream.  Template over dimension D, CG vector type T and coarse-level
+// storage type S.  All launches are asynchronous on the context's stream; the
+// host waits only when it polls Ctrl::done between batches of iterations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+#include "../../include/uaamg.h"
+#include "kernels_cg.hpp"
+#include "kernels_march.hpp"
+
+namespace uaamg {
+
+struct HipError {
+    hipError_t e; std::string where;
+};
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) throw HipError{_e, std::string(#call) + " @" + std::to_string(__LINE__)}; \
+    } while (0)
+
+struct Launch {
+    dim3 grid, block;
+    int kz;
+    long nblocks() const { return (long)grid.x * grid.y * grid.z; }
+};
+
+inline Launch cells_launch(const Geo& g, int D) {
+    Launch L;
+    L.block = dim3(64, 4, 1);
+    unsigned gx = (g.nx + 63) / 64, gy = (g.ny + 3) / 4;
+    long planes = (D == 3) ? g.nz : 1;
+    long per_plane = (long)gx * gy;
+    int kz = (int)std::max<long>(1, (per_plane * planes + 8191) / 8192);
+    if (D != 3) kz = 1;
+    unsigned gz = (D == 3) ? (unsigned)((g.nz + kz - 1) / kz) : 1u;
+    L.grid = dim3(gx, gy, gz);
+    L.kz = kz;
+    return L;
+}
+
+// Per-class device timing with events (opts.timing).
+struct Timer {
+    bool on = false;
+    unsigned mask = 0;        // bit c set: time kernel class c
+    hipStream_t s = nullptr;
+    struct Rec { int cls; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t get() {
+        if (used == pool.size()) { hipEvent_t e; HIPCHK(hipEventCreate(&e)); pool.push_back(e); }
+        return pool[used++];
+    }
+    bool want(int cls) const { return on && ((mask >> cls) & 1u); }
+    void begin(int cls, hipEvent_t& a) { if (!want(cls)) { a = nullptr; return; } a = get(); HIPCHK(hipEventRecord(a, s)); recs.push_back({cls, a, nullptr}); }
+    void end(hipEvent_t a) { if (!a) return; hipEvent_t b = get(); HIPCHK(hipEventRecord(b, s)); recs.back().b = b; }
+    void reset() { recs.clear(); used = 0; }
+    void collect(uaamg_stats* st) {
+        if (!on || !st) return;
+        for (auto& r : recs) {
+            float ms = 0; HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            st->ms_kernel[r.cls] += ms; st->n_kernel[r.cls] += 1;
+        }
+    }
+    ~Timer() { for (auto e : pool) (void)hipEventDestroy(e); }
+};
+
+struct TScope {
+    Timer& t; hipEvent_t a{};
+    TScope(Timer& t_, int cls) : t(t_) { t.begin(cls, a); }
+    ~TScope() { t.end(a); }
+};
+
+class EngineBase {
+public:
+    virtual ~EngineBase() = default;
+    virtual void set_grid(const uint8_t* solid_dev) = 0;
+    virtual int solve(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, uaamg_stats* st) = 0;
+    virtual void apply(const double* r_dev, double* z_dev) = 0;
+    virtual double matvec(const double* v_dev, double* Av_dev) = 0;
+    virtual void level_data(int level, uint8_t* solid, double* diag, double* cx, double* cy, double* cz, double* ex) = 0;
+    virtual int nlevels() const = 0;
+    virtual Geo geo(int l) const = 0;
+    virtual size_t bytes() const = 0;
+    virtual void set_opts(const uaamg_opts& o) = 0;
+    virtual void set_stream(hipStream_t s) = 0;
+};
+
+// Hierarchy shape, shared by engines and the byte estimator (buildHierarchy,
+// uaamg_preconditioner_3d.cpp:11-32): push, stop once a side <= S, halve.
+inline std::vector<Geo> hierarchy_shape(int dim, int nx, int ny, int nz, int S) {
+    std::vector<Geo> v;
+    while (nx >= 2 && ny >= 2 && (dim == 2 || nz >= 2)) {
+        v.push_back(make_geo(dim, nx, ny, nz));
+        if (nx <= S || ny <= S || (dim == 3 && nz <= S)) break;
+        nx /= 2; ny /= 2; if (dim == 3) nz /= 2;
+        if (v.size() > 40) break;
+    }
+    return v;
+}
+
+template <typename CT>
+Consts<CT> make_consts(int dim, const double h[3], const int wall[6]) {
+    Consts<CT> c{};
+    double ix = 1.0 / (h[0] * h[0]), iy = 1.0 / (h[1] * h[1]), iz = (dim == 3) ? 1.0 / (h[2] * h[2]) : 0.0;
+    c.idx2 = (CT)ix; c.idy2 = (CT)iy; c.idz2 = (CT)iz;
+    c.diag_u = (CT)((dim == 3) ? 2.0 * (ix + iy + iz) : 2.0 * (ix + iy));
+    double e[6] = {2.0 * ix, 2.0 * ix, 2.0 * iy, 2.0 * iy, 2.0 * iz, 2.0 * iz};
+    c.any_dirichlet = 0;
+    for (int f = 0; f < 6; f++) {
+        int on = (f < 2 * dim) ? (wall[f] == UAAMG_WALL_DIRICHLET) : 0;
+        c.wall[f] = on;
+        c.ex2[f] = on ? (CT)e[f] : CT(0);
+        c.any_dirichlet |= on;
+    }
+    return c;
+}
+
+template <int D, typename T, typename S>
+class Engine final : public EngineBase {
+    using CT = typename CTOf<T>::type;
+    using SC = typename CTOf<S>::type;
+    using C4 = typename Vec4<S>::type;
+
+    struct Level {
+        Geo g;
+        Launch lc;
+        S* b = nullptr; S* x = nullptr;
+        uint8_t* mask = nullptr;     // solid mask (set-up only)
+        C4* coef = nullptr;          // {cx, cy, cz, diag (< 0: solid)}
+        S* ex = nullptr;             // Dirichlet diagonal excess (set-up only)
+    };
+
+public:
+    Engine(const uaamg_opts& o, const int n[3], const double h[3], const int wall[6], hipStream_t s)
+        : o_(o), stream_(s) {
+        for (int q = 0; q < 3; q++) { n_[q] = n[q]; h_[q] = h[q]; }
+        for (int f = 0; f < 6; f++) wall_[f] = wall[f];
+        shape_ = hierarchy_shape(D, n[0], n[1], D == 3 ? n[2] : 1, o.coarsest_side);
+        if (shape_.empty()) throw HipError{hipErrorInvalidValue, "grid too small for a hierarchy"};
+        c0_ = make_consts<CT>(D, h, wall);
+        c0d_ = make_consts<double>(D, h, wall);
+        g0_ = shape_[0];
+        N4_ = (g0_.N + 3) / 4 * 4;
+        margin_ = ((g0_.sx + 8 + 63) / 64) * 64;
+        lc0_ = cells_launch(g0_, D);
+        int all_neumann = 1;
+        for (int f = 0; f < 2 * D; f++) if (wall[f] != UAAMG_WALL_NEUMANN) all_neumann = 0;
+        project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
+        timer_.s = stream_;
+        allocate();
+    }
+    ~Engine() override { release(); }
+
+    void set_opts(const uaamg_opts& o) override {
+        int all_neumann = 1;
+        for (int f = 0; f < 2 * D; f++) if (wall_[f] != UAAMG_WALL_NEUMANN) all_neumann = 0;
+        o_.nu_pre = o.nu_pre; o_.nu_post = o.nu_post; o_.coarsest_sweeps = o.coarsest_sweeps;
+        o_.tol_mode = o.tol_mode; o_.max_iter = o.max_iter; o_.tol = o.tol; o_.fused = o.fused;
+        o_.check_every = o.check_every; o_.use_graph = o.use_graph; o_.timing = o.timing; o_.flexible = o.flexible;
+        o_.project_mean = o.project_mean;
+        project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
+    }
+    void set_stream(hipStream_t s) override { stream_ = s; timer_.s = s; }
+    int nlevels() const override { return (int)shape_.size(); }
+    Geo geo(int l) const override { return shape_[l]; }
+    size_t bytes() const override { return bytes_; }
+
+    // ---------------- set-up ----------------
+    void set_grid(const uint8_t* solid_dev) override {
+        const int nl = nlevels();
+        HIPCHK(hipMemcpyAsync(solid0_, solid_dev, g0_.N, hipMemcpyDeviceToDevice, stream_));
+        k_nbr0<D><<<lc0_.grid, lc0_.block, 0, stream_>>>(solid0_, nbr0_, g0_, lc0_.kz, partials_);
+        k_fin_count<<<1, 1024, 0, stream_>>>(partials_, (int)lc0_.nblocks(), count_);
+        HIPCHK(hipGetLastError());
+        for (int l = 1; l < nl; l++) {
+            Level& F = lv_[l - 1]; Level& C = lv_[l];
+            const uint8_t* fm = (l == 1) ? nbr0_ : F.mask;
+            k_coarsen_mask<D><<<C.lc.grid, C.lc.block, 0, stream_>>>(fm, C.mask, F.g, C.g, C.lc.kz, l == 1);
+            HIPCHK(hipGetLastError());
+            if (l == 1) {
+                FineL0 src{nbr0_, c0d_, g0_};
+                k_galerkin<D, FineL0, C4, S><<<C.lc.grid, C.lc.block, 0, stream_>>>(src, F.g, C.mask, C.coef, C.ex, C.g, C.lc.kz);
+            } else {
+                FineLc<C4, S> src{F.coef, F.ex};
+                k_galerkin<D, FineLc<C4, S>, C4, S><<<C.lc.grid, C.lc.block, 0, stream_>>>(src, F.g, C.mask, C.coef, C.ex, C.g, C.lc.kz);
+            }
+            k_level_diag<D, C4, S><<<C.lc.grid, C.lc.block, 0, stream_>>>(C.coef, C.ex, C.mask, C.g, C.lc.kz);
+            HIPCHK(hipGetLastError());
+        }
+        graph_valid_ = false;
+    }
+
+    // ---------------- V-cycle ----------------
+    // Level 0 right-hand side is the CG residual r, its solution is z.
+    void vcycle(const T* r, T* z, const int* skip) {
+        zsums_fused_ = o_.fused && o_.nu_pre == 1 && o_.nu_post == 1 && fused_ok_;
+        if (zsums_fused_) vcycle_fused(r, z, skip);
+        else vcycle_pass(r, z, skip);
+    }
+
+    void vcycle_pass(const T* r, T* z, const int* skip) {
+        const int nl = nlevels();
+        fill_async(z, g0_.N, skip);
+        for (int l = 1; l < nl; l++) fill_async(lv_[l].x, lv_[l].g.N, skip);
+        LevelArgs<T, C4> L0{g0_, const_cast<T*>(r), z, nbr0_, nullptr};
+        if (nl == 1) { coarsest_pass<true>(L0, skip); return; }
+        // down
+        for (int l = 0; l < nl - 1; l++) {
+            TScope ts(timer_, l == 0 ? UAAMG_K_DOWN0 : UAAMG_K_COARSE);
+            Level& C = lv_[l + 1];
+            LevelArgs<S, C4> LC{C.g, C.b, C.x, C.mask, C.coef};
+            if (l == 0) {
+                smooth_pass<true>(L0, o_.nu_pre, false, skip);
+                k_restrict_skip<D, true>(L0, LC, C.lc, skip);
+            } else {
+                Level& F = lv_[l];
+                LevelArgs<S, C4> LF{F.g, F.b, F.x, F.mask, F.coef};
+                smooth_pass<false>(LF, o_.nu_pre, false, skip);
+                k_restrict_skip<D, false>(LF, LC, C.lc, skip);
+            }
+        }
+        {
+            TScope ts(timer_, UAAMG_K_COARSE);
+            Level& C = lv_[nl - 1];
+            LevelArgs<S, C4> LC{C.g, C.b, C.x, C.mask, C.coef};
+            coarsest_pass<false>(LC, skip);
+        }
+        for (int l = nl - 2; l >= 0; l--) {
+            TScope ts(timer_, l == 0 ? UAAMG_K_UP0 : UAAMG_K_COARSE);
+            Level& C = lv_[l + 1];
+            LevelArgs<S, C4> LC{C.g, C.b, C.x, C.mask, C.coef};
+            if (l == 0) {
+                prolong_skip<true>(L0, LC, skip);
+                smooth_pass<true>(L0, o_.nu_post, true, skip);
+            } else {
+                Level& F = lv_[l];
+                LevelArgs<S, C4> LF{F.g, F.b, F.x, F.mask, F.coef};
+                prolong_skip<false>(LF, LC, skip);
+                smooth_pass<false>(LF, o_.nu_post, true, skip);
+            }
+        }
+    }
+
+    // ---- fused V-cycle: one down and one up launch per level, LDS coarsest ----
+    // 2.5D marching kernels: in-plane region RX x RY (tile + 2-ring), PPT
+    // positions per thread, march over KC planes of the slowest axis
+    static constexpr int RX0 = (D == 3) ? 64 : 256, RY0 = (D == 3) ? 16 : 1, PPT0 = (D == 3) ? 4 : 1;
+    static constexpr int RXC = (D == 3) ? 32 : 256, RYC = (D == 3) ? 8 : 1, PPTC = 1;
+    static constexpr int CMAX = 2048;
+
+    struct MLaunch { dim3 grid; int KC; };
+    static MLaunch march_launch(const Geo& g, int RX, int RY, long target) {
+        int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1;
+        unsigned ta = (unsigned)((g.nx + TX - 1) / TX), tb = (D == 3) ? (unsigned)((g.ny + TY - 1) / TY) : 1u;
+        int nm = (D == 3) ? g.nz : g.ny;
+        long tiles = (long)ta * tb;
+        long chunks = std::max<long>(1, target / std::max<long>(1, tiles));
+        int KC = (int)((nm + chunks - 1) / chunks);
+        KC = std::max(KC, 8);
+        KC += (KC & 1);
+        MLaunch m;
+        m.KC = KC;
+        m.grid = dim3(ta, tb, (unsigned)((nm + KC - 1) / KC));
+        return m;
+    }
+    long fused_max_blocks() const {
+        MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+        return (long)m.grid.x * m.grid.y * m.grid.z;
+    }
+    bool fused_supported() const { return nlevels() >= 2; }
+
+    void vcycle_fused(const T* r, T* z, const int* skip) {
+        const int nl = nlevels();
+        FusedArgs<D, true, CT, C4> A0{g0_, nbr0_, nullptr, c0_};
+        for (int l = 0; l < nl - 1; l++) {
+            Level& C = lv_[l + 1];
+            if (l == 0) {
+                TScope ts(timer_, UAAMG_K_DOWN0);
+                MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+                k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
+                    r, A0, C.b, C.coef, C.g, m.KC, skip);
+            } else {
+                TScope ts(timer_, UAAMG_K_COARSE);
+                Level& F = lv_[l];
+                FusedArgs<D, false, SC, C4> A{F.g, nullptr, F.coef, Consts<SC>{}};
+                MLaunch m = march_launch(F.g, RXC, RYC, 1024);
+                k_down_m<D, false, RXC, RYC, PPTC, S, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
+                    F.b, A, C.b, C.coef, C.g, m.KC, skip);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        {
+            TScope ts(timer_, UAAMG_K_COARSE);
+            Level& C = lv_[nl - 1];
+            int npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+            if (C.g.N <= CMAX) {
+                k_coarsest<D, CMAX, S, C4><<<1, 256, 0, stream_>>>(C.x, C.b, C.coef, C.g, npass, skip);
+            } else {
+                fill_async(C.x, C.g.N, skip);
+                LevelArgs<S, C4> LC{C.g, C.b, C.x, C.mask, C.coef};
+                coarsest_pass<false>(LC, skip);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        for (int l = nl - 2; l >= 0; l--) {
+            Level& C = lv_[l + 1];
+            if (l == 0) {
+                TScope ts(timer_, UAAMG_K_UP0);
+                MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+                k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S><<<m.grid, 256, 0, stream_>>>(
+                    r, A0, C.x, C.g, z, m.KC, skip, partials_);
+                k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
+            } else {
+                TScope ts(timer_, UAAMG_K_COARSE);
+                Level& F = lv_[l];
+                FusedArgs<D, false, SC, C4> A{F.g, nullptr, F.coef, Consts<SC>{}};
+                MLaunch m = march_launch(F.g, RXC, RYC, 1024);
+                k_up_m<D, false, RXC, RYC, PPTC, false, S, C4, S, S><<<m.grid, 256, 0, stream_>>>(
+                    F.b, A, C.x, C.g, F.x, m.KC, skip, nullptr);
+            }
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    // ---------------- solve ----------------
+    int solve(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, uaamg_stats* st) override {
+        auto t0 = std::chrono::steady_clock::now();
+        timer_.on = (o_.timing != 0);
+        timer_.mask = (unsigned)o_.timing;
+        timer_.reset();
+        T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
+        {
+            TScope ts(timer_, UAAMG_K_INIT);
+            double* hist = nullptr;
+            if (trace_) {
+                if (hist_n_ < max_iter) {
+                    if (hist_) HIPCHK(hipFree(hist_));
+                    HIPCHK(hipMalloc(&hist_, sizeof(double) * std::max(1, max_iter)));
+                    hist_n_ = max_iter;
+                }
+                hist = hist_;
+            }
+            k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, max_iter, tol, o_.tol_mode, project_, o_.flexible, hist);
+            if (!x_is_p()) {
+                if (warm) k_cast_in<T><<<flat_blocks(2048), 256, 0, stream_>>>(p_dev, x_, nbr0_, g0_.N);
+                else fill_async(x_, N4_, nullptr);
+            }
+            fill_async(p_, N4_, nullptr);
+            const unsigned fb = flat_blocks(2048);
+            k_init_sum<<<fb, 256, 0, stream_>>>(rhs_dev, nbr0_, N4_, partials_, ctrl_);
+            k_fin_init_sum<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_init_r<T><<<fb, 256, 0, stream_>>>(rhs_dev, r_, nbr0_, N4_, partials_, ctrl_);
+            k_fin_init_r<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_init_sub<T><<<fb, 256, 0, stream_>>>(r_, nbr0_, N4_, partials_, ctrl_);
+            k_fin_init_sub<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            if (warm) {
+                k_init_warm<D, T><<<fb, 256, 0, stream_>>>(r_, x, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+                k_fin_init_warm<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        const int* skip = &ctrl_->done;
+        vcycle(r_, z_, skip);
+        zsums(skip);
+        pupdate();
+        int it_cap = max_iter;
+        int every = std::max(1, o_.check_every);
+        int queued = 0;
+        Ctrl hc{};
+        while (true) {
+            int chunk = std::min(every, std::max(1, it_cap - queued));
+            for (int c = 0; c < chunk; c++) iteration(x, skip);
+            queued += chunk;
+            HIPCHK(hipMemcpyAsync(ctrl_host_, ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            std::memcpy(&hc, ctrl_host_, sizeof(Ctrl));
+            if (hc.done || queued >= it_cap) break;
+        }
+        if (!x_is_p()) {
+            TScope ts(timer_, UAAMG_K_OTHER);
+            k_finalize<T><<<flat_blocks(2048), 256, 0, stream_>>>(x_, p_dev, nbr0_, g0_.N, warm);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
+        auto t1 = std::chrono::steady_clock::now();
+        if (trace_ && hc.iter > 0) {
+            std::vector<double> h(hc.iter);
+            HIPCHK(hipMemcpy(h.data(), hist_, sizeof(double) * hc.iter, hipMemcpyDeviceToHost));
+            std::fprintf(stderr, "[uaamg] r0=%.6e", std::sqrt(hc.r0sq));
+            for (int q = 0; q < hc.iter; q++) std::fprintf(stderr, "%s%.3e", q % 10 ? " " : "\n[uaamg] ", std::sqrt(h[q] / hc.r0sq));
+            std::fprintf(stderr, "\n");
+        }
+        if (st) {
+            st->iters = hc.iter;
+            st->status = hc.status == ST_CONVERGED ? UAAMG_OK
+                       : hc.status == ST_BREAKDOWN ? UAAMG_BREAKDOWN
+                       : hc.status == ST_NONFINITE ? UAAMG_NONFINITE : UAAMG_NOT_CONVERGED;
+            st->nlevels = nlevels();
+            st->rnorm = std::sqrt(std::max(0.0, hc.rsnew));
+            st->r0norm = std::sqrt(std::max(0.0, hc.r0sq));
+            st->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            timer_.collect(st);
+            st->alg_bytes = alg_bytes(hc.iter);
+        }
+        return st ? st->status : 0;
+    }
+
+    // persistent grid for the flat level-0 kernels: at most `cap` workgroups of 256
+    unsigned flat_blocks(unsigned cap) const {
+        long need = (N4_ / 4 + 255) / 256;
+        return (unsigned)std::max<long>(1, std::min<long>(need, cap));
+    }
+
+    void iteration(T* x, const int* skip) {
+        const unsigned fb = flat_blocks(2048);
+        {
+            TScope ts(timer_, UAAMG_K_MATVEC);
+            k_matvec_dot<D, T><<<fb, 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+            k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+        }
+        {
+            TScope ts(timer_, UAAMG_K_UPDATE);
+            if (o_.flexible) {
+                k_update<T, true><<<fb, 256, 0, stream_>>>(x, r_, p_, Ap_, z_, nbr0_, N4_, partials_, ctrl_);
+                k_fin_update<true><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            } else {
+                k_update<T, false><<<fb, 256, 0, stream_>>>(x, r_, p_, Ap_, z_, nbr0_, N4_, partials_, ctrl_);
+                k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            }
+        }
+        vcycle(r_, z_, skip);
+        zsums(skip);
+        pupdate();
+    }
+
+    void zsums(const int*) {
+        if (zsums_fused_) return;   // the fused level-0 up kernel already reduced
+        TScope ts(timer_, UAAMG_K_PUPDATE);
+        const unsigned fb = flat_blocks(2048);
+        k_zsums<T><<<fb, 256, 0, stream_>>>(z_, r_, nbr0_, N4_, partials_, ctrl_);
+        k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+    }
+    void pupdate() {
+        TScope ts(timer_, UAAMG_K_PUPDATE);
+        k_pupdate<T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, z_, nbr0_, N4_, ctrl_);
+    }
+
+    // ---------------- test hooks ----------------
+    void apply(const double* r_dev, double* z_dev) override {
+        k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, 1, 0.0, 0, project_, 0, nullptr);
+        cast_in_all(r_dev, r_);
+        vcycle(r_, z_, nullptr);   // the fused level-0 reductions land in ctrl, unused here
+        cast_out_all(z_, z_dev);
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+
+    double matvec(const double* v_dev, double* Av_dev) override {
+        k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, 1, 0.0, 0, project_, 0, nullptr);
+        cast_in_all(v_dev, p_);
+        k_matvec_dot<D, T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)flat_blocks(2048), ctrl_);
+        cast_out_all(Ap_, Av_dev);
+        HIPCHK(hipMemcpyAsync(ctrl_host_, ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        Ctrl hc; std::memcpy(&hc, ctrl_host_, sizeof(Ctrl));
+        return hc.pAp;
+    }
+
+    void level_data(int level, uint8_t* solid, double* diag, double* cx, double* cy, double* cz, double* ex) override;
+
+private:
+    // the iterate always lives in an internal buffer (guard margins, aligned);
+    // the caller's p receives p += x (cold start) or p = x (warm start) at the end
+    bool x_is_p() const { return false; }
+
+    // zero-fill with a kernel (no memset nodes, so the sequence can be graph-captured)
+    template <typename V>
+    void fill_async(V* p, long n, const int* skip) {
+        long blocks = std::min<long>((n + 255) / 256, 4096);
+        k_fill<V><<<(unsigned)std::max<long>(blocks, 1), 256, 0, stream_>>>(p, n, V(0), skip);
+        HIPCHK(hipGetLastError());
+    }
+
+    template <bool L0, typename V>
+    void smooth_pass(LevelArgs<V, C4>& A, int sweeps, bool reverse, const int* skip) {
+        Launch lc = cells_launch(A.g, D);
+        int p0 = reverse ? 0 : 1;
+        for (int sw = 0; sw < sweeps; sw++) {
+            k_smooth_skip<L0>(A, p0, lc, skip);
+            k_smooth_skip<L0>(A, 1 - p0, lc, skip);
+        }
+    }
+    template <bool L0, typename V>
+    void coarsest_pass(LevelArgs<V, C4>& A, const int* skip) {
+        for (int s = 0; s < o_.coarsest_sweeps; s++) {
+            smooth_pass<L0>(A, 1, false, skip);
+            smooth_pass<L0>(A, 1, true, skip);
+        }
+    }
+    template <bool L0, typename V>
+    void k_smooth_skip(LevelArgs<V, C4>& A, int parity, const Launch& lc, const int* skip) {
+        k_smooth<D, L0, V, C4><<<lc.grid, lc.block, 0, stream_>>>(A, consts_for<V>(), parity, lc.kz, skip);
+        HIPCHK(hipGetLastError());
+    }
+    template <int DD, bool L0, typename V>
+    void k_restrict_skip(LevelArgs<V, C4>& F, LevelArgs<S, C4>& Cl, const Launch& lc, const int* skip) {
+        k_restrict<DD, L0, V, C4, S, C4><<<lc.grid, lc.block, 0, stream_>>>(F, Cl, consts_for<V>(), lc.kz, skip);
+        HIPCHK(hipGetLastError());
+    }
+    template <bool L0, typename V>
+    void prolong_skip(LevelArgs<V, C4>& F, LevelArgs<S, C4>& Cl, const int* skip) {
+        Launch lc = cells_launch(F.g, D);
+        k_prolong<D, L0, V, C4, S, C4><<<lc.grid, lc.block, 0, stream_>>>(F, Cl, lc.kz, skip);
+        HIPCHK(hipGetLastError());
+    }
+    template <typename V>
+    Consts<typename CTOf<V>::type> consts_for() const {
+        if constexpr (std::is_same<typename CTOf<V>::type, CT>::value) return c0_;
+        else return make_consts<typename CTOf<V>::type>(D, h_, wall_);
+    }
+
+    void cast_in_all(const double* src, T* dst);
+    void cast_out_all(const T* src, double* dst);
+
+    double alg_bytes(int iters) const {
+        // canonical count of the reference algorithm's passes:
+        // per iteration 13 s_v + 2 bytes (CG body) + level-0 V-cycle + coarse levels.
+        double sv = sizeof(T), sl = sizeof(S), sc = sizeof(S);
+        double rho = (D == 3) ? 1.0 / 8.0 : 1.0 / 4.0, ch = (D == 3) ? 4 : 3;
+        double N0 = (double)n_[0] * n_[1] * (D == 3 ? n_[2] : 1);
+        double cg = 13 * sv + 2;
+        double l0 = 3 * sv + 2 * sl + 2 + 2 * rho * sl;
+        double coarse = (5 * sl + 2 + 2 * ch * sc + 2 * rho * sl) * (rho / (1 - rho));
+        double per_iter = cg + l0 + coarse;
+        double setup = 3 * sv + 3 + (1 + ch * sc) * (rho / (1 - rho));
+        double init = 48;
+        return N0 * (setup + init + iters * per_iter);
+    }
+
+    void allocate();
+    void release();
+
+    uaamg_opts o_;
+    hipStream_t stream_;
+    int n_[3]; double h_[3]; int wall_[6];
+    int project_ = 1;
+    std::vector<Geo> shape_;
+    Geo g0_;
+    long N4_ = 0;          // level-0 padded length rounded up to a multiple of 4 (flat vector kernels)
+    Launch lc0_;
+    Consts<CT> c0_;
+    Consts<double> c0d_;
+    std::vector<Level> lv_;
+    uint8_t* solid0_ = nullptr; uint8_t* nbr0_ = nullptr;
+    T *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *Ap_ = nullptr, *x_ = nullptr;
+    double* partials_ = nullptr; double* count_ = nullptr;
+    Ctrl* ctrl_ = nullptr; Ctrl* ctrl_host_ = nullptr;
+    size_t bytes_ = 0;
+    bool graph_valid_ = false;
+    bool fused_ok_ = false;
+    bool zsums_fused_ = false;
+    Timer timer_;
+    std::vector<void*> allocs_;
+    bool trace_ = std::getenv("UAAMG_TRACE") != nullptr;   // relres history to stderr (cf. PCG_DBG, cuda_pcg_3d_opt.cu:384)
+    double* hist_ = nullptr; int hist_n_ = 0;
+
+    // level-0 vectors carry a zeroed guard margin on both sides: the 4-cell
+    // stencil loads of k_matvec_dot / k_init_warm touch up to sx+4 cells past
+    // the first/last interior cell (2D first/last rows)
+    long margin_ = 0;
+    template <typename P> P* dalloc_margin(size_t n) {
+        P* base = dalloc<P>(n + 2 * (size_t)margin_);
+        return base + margin_;
+    }
+    template <typename P> P* dalloc(size_t n) {
+        void* p = nullptr;
+        size_t bytes = n * sizeof(P);
+        if (bytes == 0) bytes = 8;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+        allocs_.push_back(p);
+        bytes_ += bytes;
+        HIPCHK(hipMemsetAsync(p, 0, bytes, stream_));
+        return reinterpret_cast<P*>(p);
+    }
+};
+
+}  // namespace uaamg
