@@ -118,7 +118,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "independent solves shard per GPU with no communication, see DESIGN.md.", file=sys.stderr)
+              "independent solves shard per GPU with no communication; a single solve would need z-slab halos.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -534,7 +534,7 @@ private:
     void cast_out_all(const T* src, double* dst);
 
     double alg_bytes(int iters) const {
-        // canonical count of the reference algorithm's passes:
+        // canonical count of the reference algorithm's logical passes, each array read or written once per pass:
         // per iteration 13 s_v + 2 bytes (CG body) + level-0 V-cycle + coarse levels.
         double sv = sizeof(T), sl = sizeof(S), sc = sizeof(S);
         double rho = (D == 3) ? 1.0 / 8.0 : 1.0 / 4.0, ch = (D == 3) ? 4 : 3;
