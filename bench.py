@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (3x), the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -131,6 +134,14 @@ def main():
     log(f"[bench] generated {pb.name} in {time.perf_counter() - t0:.1f}s, solid fraction {pb.solid.mean():.3f}")
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 x copy of {src.numel() * 4} bytes (read {src.numel() * 4}, write {src.numel() * 4})")
+        del src, dst
     rhs_d = torch.from_numpy(pb.rhs).to(dev)
     solid_d = torch.from_numpy(pb.solid).to(dev)
     p_d = torch.zeros(pb.N, dtype=torch.float64, device=dev)

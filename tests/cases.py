@@ -78,4 +78,19 @@ def parity_cases():
     pb = ob.Problem((24, 20, 18), (1.0, 1.1, 0.7))
     pb.solid = random_solids(pb, rng)
     out.append(("3d_24x20x18_aniso_solids", pb))
+    # several workgroup tiles in y and z, mixed Dirichlet / Neumann walls
+    pb = ob.Problem((40, 36, 34), (1.0, 0.9, 0.85),
+                    wall=[ob.DIRICHLET, ob.NEUMANN, ob.NEUMANN, ob.DIRICHLET, ob.DIRICHLET, ob.NEUMANN])
+    pb.solid = random_solids(pb, rng)
+    out.append(("3d_40x36x34_walls_solids", pb))
+    # x-rows longer than 128 cells (partly filled last lanes)
+    pb = ob.Problem((130, 20, 12), (1.3, 0.2, 0.12))
+    pb.solid = balls(pb, [(0.4, 0.1, 0.06)], [0.05])
+    out.append(("3d_130x20x12", pb))
+    out.append(("2d_40x30_dirichlet", ob.Problem((40, 30), (1.0, 0.75), wall=[ob.DIRICHLET] * 6)))
     return out
+
+
+def neumann_cases():
+    """parity_cases() with all walls Neumann (the reference's own setting)."""
+    return [(n, pb) for n, pb in parity_cases() if all(w == ob.NEUMANN for w in pb.wall)]

@@ -15,6 +15,7 @@
 #include "../../include/uaamg.h"
 #include "kernels_cg.hpp"
 #include "kernels_march.hpp"
+#include "kernels_rows.hpp"
 
 namespace uaamg {
 
@@ -157,6 +158,7 @@ public:
         int all_neumann = 1;
         for (int f = 0; f < 2 * D; f++) if (wall[f] != UAAMG_WALL_NEUMANN) all_neumann = 0;
         project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
+        rows_setup(h, wall);
         timer_.s = stream_;
         allocate();
     }
@@ -274,6 +276,69 @@ public:
         m.grid = dim3(ta, tb, (unsigned)((nm + KC - 1) / KC));
         return m;
     }
+    // ---- level-0 row kernels (3D, FP32 vectors, nx <= 256) ----
+    static constexpr int ROWS_NW = 10, ROWS_RW = 2;
+    int rows_cpl_ = 0;          // cells per lane (0: row kernels not used)
+    RowTab rtab_{};
+    RowGeo rgeo_{};
+    unsigned rows_blocks_ = 0;
+    void rows_setup(const double h[3], const int wall[6]) {
+        rows_cpl_ = 0;
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            const char* env = std::getenv("UAAMG_ROWS");
+            if (env && env[0] == '0') return;
+            if (nlevels() < 2) return;
+            const int nx = n_[0];
+            rows_cpl_ = nx <= 128 ? 2 : (nx <= 256 ? 4 : 0);
+            if (!rows_cpl_) return;
+            const double ix = 1.0 / (h[0] * h[0]), iy = 1.0 / (h[1] * h[1]), iz = 1.0 / (h[2] * h[2]);
+            for (int m = 0; m < 64; m++) {
+                double d = ix * (((m >> 0) & 1) + ((m >> 1) & 1)) + iy * (((m >> 2) & 1) + ((m >> 3) & 1))
+                         + iz * (((m >> 4) & 1) + ((m >> 5) & 1));
+                rtab_.d[m] = (float)d;
+                rtab_.rinv[m] = d >= 1e-30 ? (float)(1.0 / d) : 0.0f;
+            }
+            rtab_.idx2 = (float)ix; rtab_.idy2 = (float)iy; rtab_.idz2 = (float)iz;
+            const double e[6] = {2 * ix, 2 * ix, 2 * iy, 2 * iy, 2 * iz, 2 * iz};
+            rtab_.any_dirichlet = 0;
+            for (int f = 0; f < 6; f++) {
+                const bool on = wall[f] == UAAMG_WALL_DIRICHLET;
+                rtab_.ex[f] = on ? (float)e[f] : 0.0f;
+                rtab_.any_dirichlet |= on;
+            }
+            constexpr int TY = ROWS_NW * ROWS_RW - 4;
+            int KC = 16;
+            if (const char* kc = std::getenv("UAAMG_ROWS_KC")) KC = std::max(4, std::atoi(kc) / 4 * 4);
+            rgeo_.KC = KC;
+            rgeo_.nty = (n_[1] + TY - 1) / TY;
+            rgeo_.nkc = (n_[2] + KC - 1) / KC;
+            rgeo_.ntiles = rgeo_.nty * rgeo_.nkc;
+            rgeo_.per_xcd = (rgeo_.ntiles + 7) / 8;
+            rows_blocks_ = 8u * (unsigned)rgeo_.per_xcd;
+        }
+    }
+    void rows_down0(const T* r, Level& C, const int* skip) {
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            if (rows_cpl_ == 4)
+                k_rows_down0<4, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
+                    r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
+            else
+                k_rows_down0<2, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
+                    r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
+        }
+    }
+    void rows_up0(const T* r, Level& C, T* z, const int* skip) {
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            if (rows_cpl_ == 4)
+                k_rows_up0<4, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
+                    r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
+            else
+                k_rows_up0<2, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
+                    r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
+            k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+        }
+    }
+
     long fused_max_blocks() const {
         MLaunch m = march_launch(g0_, RX0, RY0, 1024);
         return (long)m.grid.x * m.grid.y * m.grid.z;
@@ -285,7 +350,10 @@ public:
         FusedArgs<D, true, CT, C4> A0{g0_, nbr0_, nullptr, c0_};
         for (int l = 0; l < nl - 1; l++) {
             Level& C = lv_[l + 1];
-            if (l == 0) {
+            if (l == 0 && rows_cpl_) {
+                TScope ts(timer_, UAAMG_K_DOWN0);
+                rows_down0(r, C, skip);
+            } else if (l == 0) {
                 TScope ts(timer_, UAAMG_K_DOWN0);
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
                 k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
@@ -315,7 +383,10 @@ public:
         }
         for (int l = nl - 2; l >= 0; l--) {
             Level& C = lv_[l + 1];
-            if (l == 0) {
+            if (l == 0 && rows_cpl_) {
+                TScope ts(timer_, UAAMG_K_UP0);
+                rows_up0(r, C, z, skip);
+            } else if (l == 0) {
                 TScope ts(timer_, UAAMG_K_UP0);
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
                 k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S><<<m.grid, 256, 0, stream_>>>(

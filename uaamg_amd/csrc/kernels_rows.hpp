@@ -1,0 +1,456 @@
+// kernels_rows.hpp -- level-0 V(1,1) kernels of the FP32 path (3D).
+//
+// The same preconditioner as the reference's level-0 smooth /
+// restrictResidual / prolongateAdd (uaamg_preconditioner_3d.cpp:123-181).
+// The level's x starts at 0 in every V-cycle, so the forward sweep and the
+// residual are local formulas of r (the level-0 b):
+//   odd  cells: x0 = r / d                      (forward sweep, odd colour)
+//   even cells: x0 = (r + sum_nb c x0_nb) / d   (forward sweep, even colour)
+//   residual:   odd: sum_nb c x0_nb  (r - d x0 = 0),  even: 0,
+//               isolated fluid cells (d < 1e-30, skipped by the sweeps): r
+// and on the way up, with x' = x0 + 2 x_coarse on fluid cells:
+//   even cells: z = (r + sum_nb c x'_nb) / d    (reverse sweep, even colour)
+//   odd  cells: z = (r + sum_nb c z_nb) / d     (reverse sweep, odd colour)
+//   isolated cells keep x'.
+// These are identities in exact arithmetic; the FP32 path rounds them
+// differently from the FP64 parity kernels (which keep the reference order).
+// Level-0 couplings are 1/h^2 on every face between two fluid interior cells
+// and 0 otherwise, and x is 0 on every non-fluid cell, so the neighbour sums
+// use the per-axis constants.
+//
+// Mapping (wave64): a wave owns whole x-rows, CPL consecutive cells per lane
+// (nx <= 64*CPL, no x halo); RW rows per wave, NW waves stacked in y (2 halo
+// rows on each side of the tile); the workgroup marches along z over a chunk
+// of KC planes (+2 halo planes each side).  Colours are compile-time per
+// (cell, row, plane slot), so no lane idles on the other colour.  x-neighbours
+// across lanes move by wave shuffle, y-neighbours across waves through LDS
+// (one barrier per plane step), z-neighbours stay in a 4-plane register
+// queue.  The 2x2x2 restriction children all live in one thread.  Workgroups
+// are numbered XCD-first (consecutive ids go to different XCDs), so the tiles
+// one XCD runs are neighbours in y and share their halo rows in its L2.
+#pragma once
+#include <type_traits>
+#include "common.hpp"
+
+namespace uaamg {
+
+// Level-0 stencil constants of the FP32 path: 1/d and d per neighbour pattern
+// (nb >> 1 = +x,-x,+y,-y,+z,-z present; 1/d = 0 where d < 1e-30).  Dirichlet
+// walls add their excess per cell.
+struct RowTab {
+    float rinv[64];
+    float d[64];
+    float idx2, idy2, idz2;
+    float ex[6];
+    int any_dirichlet;
+};
+
+struct RowGeo {
+    int nty;       // tiles in y
+    int nkc;       // chunks in z
+    int KC;        // planes per chunk (multiple of 4)
+    int ntiles;    // nty * nkc
+    int per_xcd;   // ceil(ntiles / 8)
+};
+
+__device__ __forceinline__ float lane_from_left(float v) {   // lane l gets lane l-1; lane 0 gets 0
+    float u = __shfl_up(v, 1, 64);
+    return (threadIdx.x & 63) == 0 ? 0.0f : u;
+}
+__device__ __forceinline__ float lane_from_right(float v) {  // lane l gets lane l+1; lane 63 gets 0
+    float u = __shfl_down(v, 1, 64);
+    return (threadIdx.x & 63) == 63 ? 0.0f : u;
+}
+
+// tile of this workgroup: consecutive block ids go round-robin to the 8 XCDs,
+// so block b runs on XCD b % 8; XCD x gets tiles [x*per_xcd, (x+1)*per_xcd)
+__device__ __forceinline__ int row_tile(const RowGeo& rg, int& ty, int& kc) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * rg.per_xcd + (b >> 3);
+    ty = t % rg.nty;
+    kc = t / rg.nty;
+    return t;
+}
+
+__device__ __forceinline__ float rows_rinv(uint8_t nb, const float* srinv, const float* sd, const RowTab& T,
+                                           const Geo& g, int i, int j, int k) {
+    if (!(nb & NB_FLUID)) return 0.0f;
+    if (!T.any_dirichlet) return srinv[nb >> 1];
+    float d = sd[nb >> 1];
+    if (i == 1) d += T.ex[0];
+    if (i == g.nx) d += T.ex[1];
+    if (j == 1) d += T.ex[2];
+    if (j == g.ny) d += T.ex[3];
+    if (k == 1) d += T.ex[4];
+    if (k == g.nz) d += T.ex[5];
+    return d >= 1e-30f ? 1.0f / d : 0.0f;
+}
+
+template <int CPL, int NW, int RW>
+struct RowLDS {
+    float tab[2][64];
+    // rows shared between waves: [buffer][kind 0: S1 output, 1: S2 output]
+    // [wave][0: first row, 1: last row][colour cell q/2][lane]
+    float ex[2][2][NW][2][CPL / 2][64];
+};
+
+template <int V> using IC = std::integral_constant<int, V>;
+
+// Sum over the 6 neighbours of cell (T, Q) of plane slot s: in-plane values
+// from X[s] (other waves' rows from the LDS exchange eb/kind), z-neighbours
+// from slots sp and sm; edge = the neighbour across the lane boundary.
+template <int CPL, int NW, int RW, int T, int Q, int s, int sp, int sm>
+__device__ __forceinline__ float row_nsum(const float (&X)[4][RW][CPL], float edge, const RowLDS<CPL, NW, RW>& L,
+                                          int eb, int kind, int w, int lane, const RowTab& K) {
+    float xl, xr, xu, xd;
+    if constexpr (Q > 0) xl = X[s][T][Q - 1]; else xl = edge;
+    if constexpr (Q < CPL - 1) xr = X[s][T][Q + 1]; else xr = edge;
+    if constexpr (T > 0) xu = X[s][T - 1][Q]; else xu = L.ex[eb][kind][w > 0 ? w - 1 : 0][1][Q / 2][lane];
+    if constexpr (T < RW - 1) xd = X[s][T + 1][Q]; else xd = L.ex[eb][kind][w < NW - 1 ? w + 1 : NW - 1][0][Q / 2][lane];
+    return K.idx2 * (xl + xr) + K.idy2 * (xu + xd) + K.idz2 * (X[sp][T][Q] + X[sm][T][Q]);
+}
+
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+    if constexpr (N > 0) {
+        sfor<N - 1>(f);
+        f(IC<N - 1>{});
+    }
+}
+
+// Shared set-up of both kernels: thread geometry and the 4-plane queues.
+template <int CPL, int NW, int RW>
+struct RowThread {
+    static constexpr int TY = NW * RW - 4;
+    int lane, w, i0, b0, m0, last;
+    int jrow[RW];
+    long off[RW][CPL];
+    bool ok[RW][CPL];
+    bool tile_row[RW];
+    __device__ void init(const Geo& g, const RowGeo& rg, int ty, int kc) {
+        lane = threadIdx.x & 63;
+        w = threadIdx.x >> 6;
+        i0 = 1 + CPL * lane;
+        b0 = 1 + ty * TY;
+        m0 = 1 + kc * rg.KC;
+        last = min(m0 + rg.KC - 1, g.nz);
+#pragma unroll
+        for (int t = 0; t < RW; t++) {
+            jrow[t] = b0 - 2 + w * RW + t;
+            tile_row[t] = (w * RW + t >= 2) && (w * RW + t < TY + 2) && jrow[t] <= g.ny;
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                ok[t][q] = (i0 + q <= g.nx) && jrow[t] >= 1 && jrow[t] <= g.ny;
+                off[t][q] = (long)(i0 + q) + (long)jrow[t] * g.sx;
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ down
+// r -> coarse b (restricted residual after the forward sweep).  A workgroup
+// writes the coarse cells whose 8 children lie in its tile.
+template <int CPL, int NW, int RW, typename VC>
+__global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
+                                                        Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
+                                                        const int* skip) {
+    static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
+    if (skip && *skip) return;
+    __shared__ RowLDS<CPL, NW, RW> L;
+    int ty, kc;
+    if (row_tile(rg, ty, kc) >= rg.ntiles) return;   // the whole workgroup leaves together
+    RowThread<CPL, NW, RW> th;
+    th.init(g, rg, ty, kc);
+    const int lane = th.lane, w = th.w;
+    for (int e = threadIdx.x; e < 64; e += NW * 64) { L.tab[0][e] = K.rinv[e]; L.tab[1][e] = K.d[e]; }
+    for (int e = threadIdx.x; e < 2 * 2 * NW * 2 * (CPL / 2) * 64; e += NW * 64) (&L.ex[0][0][0][0][0][0])[e] = 0.0f;
+    __syncthreads();
+
+    float X[4][RW][CPL], Rq[4][RW][CPL], Iq[4][RW][CPL];
+    float pfr[2][RW][CPL];
+    uint8_t pfn[2][RW][CPL];
+    float acc[RW / 2][CPL / 2];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) { X[s][t][q] = 0.0f; Rq[s][t][q] = 0.0f; Iq[s][t][q] = 0.0f; }
+#pragma unroll
+    for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+        for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
+
+    auto issue = [&](int slot, int P) {
+        const bool pin = P >= 1 && P <= g.nz;
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const bool v = pin && th.ok[t][q];
+                const long id = th.off[t][q] + (long)P * g.sy;
+                pfr[slot][t][q] = v ? r[id] : 0.0f;
+                pfn[slot][t][q] = v ? nbr[id] : (uint8_t)0;
+            }
+    };
+    issue(1, th.m0 - 2);
+    issue(0, th.m0 - 1);
+
+    // one plane step: plane P enters the window; S = P & 3
+    auto step = [&](auto Sc, int P) {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;   // planes P-1, P-2, P-3
+        constexpr int PB = S & 1;        // prefetch buffer of plane P
+        constexpr int EW = S & 1;        // LDS buffer written in this step
+        constexpr int ER = EW ^ 1;       // LDS buffer written in the previous step
+        // S1: plane P enters; odd cells x0 = r / d.  Parity of (i,j,k) is
+        // (q + t + P) & 1 (i0 and the first region row are odd, RW even).
+        sfor<RW>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            sfor<CPL>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const uint8_t nb = pfn[PB][t][q];
+                const float rv = (nb & NB_FLUID) ? pfr[PB][t][q] : 0.0f;
+                const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
+                Rq[S][t][q] = rv;
+                Iq[S][t][q] = ri;
+                if constexpr (((q + t + S) & 1) == 1) X[S][t][q] = rv * ri;
+                else X[S][t][q] = 0.0f;
+            });
+        });
+        issue(PB, P + 2);
+        sfor<CPL>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (((q + S) & 1) == 1) L.ex[EW][0][w][0][q / 2][lane] = X[S][0][q];
+            if constexpr (((q + RW - 1 + S) & 1) == 1) L.ex[EW][0][w][1][q / 2][lane] = X[S][RW - 1][q];
+        });
+        // S2: plane P-1, even cells: x0 = (r + sum c x0_odd) / d
+        if (P - 1 >= th.m0 - 1) {
+            sfor<RW>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                constexpr bool q0 = ((t + S1) & 1) == 0;   // cell 0 of this row is even
+                const float edge = q0 ? lane_from_left(X[S1][t][CPL - 1]) : lane_from_right(X[S1][t][0]);
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if constexpr (((q + t + S1) & 1) == 0) {
+                        const float s = row_nsum<CPL, NW, RW, t, q, S1, S, S2>(X, edge, L, ER, 0, w, lane, K);
+                        X[S1][t][q] = (Rq[S1][t][q] + s) * Iq[S1][t][q];
+                    }
+                });
+            });
+            sfor<CPL>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (((q + S1) & 1) == 0) L.ex[EW][1][w][0][q / 2][lane] = X[S1][0][q];
+                if constexpr (((q + RW - 1 + S1) & 1) == 0) L.ex[EW][1][w][1][q / 2][lane] = X[S1][RW - 1][q];
+            });
+        }
+        // S3: plane R = P-2: residual of the odd cells, restriction over the
+        // plane pair (R-1, R) with R even
+        const int R = P - 2;
+        if (R >= th.m0 && R <= th.last) {
+            if (R & 1) {
+#pragma unroll
+                for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+                    for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
+            }
+            sfor<RW>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                constexpr bool q0 = ((t + S2) & 1) == 1;   // cell 0 of this row is odd
+                const float edge = q0 ? lane_from_left(X[S2][t][CPL - 1]) : lane_from_right(X[S2][t][0]);
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if constexpr (((q + t + S2) & 1) == 1) {
+                        const float s = row_nsum<CPL, NW, RW, t, q, S2, S1, S3>(X, edge, L, ER, 1, w, lane, K);
+                        const float res = (Iq[S2][t][q] != 0.0f) ? s : Rq[S2][t][q];
+                        acc[t / 2][q / 2] += res;
+                    }
+                });
+            });
+            if (!(R & 1)) {
+                const int kcc = R >> 1;
+#pragma unroll
+                for (int a = 0; a < RW / 2; a++) {
+                    if (!th.tile_row[2 * a]) continue;
+                    const int jc = (th.jrow[2 * a] + 1) >> 1;
+#pragma unroll
+                    for (int c = 0; c < CPL / 2; c++) {
+                        const int ic = (th.i0 + 2 * c + 1) >> 1;
+                        if (ic <= cg.nx && jc <= cg.ny && kcc <= cg.nz)
+                            bc[(long)ic + (long)jc * cg.sx + (long)kcc * cg.sy] = (VC)acc[a][c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+    for (int P = th.m0 - 2; P <= th.last + 2; P += 4) {   // m0 - 2 = 3 (mod 4)
+        step(IC<3>{}, P);
+        if (P + 1 > th.last + 2) break;
+        step(IC<0>{}, P + 1);
+        if (P + 2 > th.last + 2) break;
+        step(IC<1>{}, P + 2);
+        if (P + 3 > th.last + 2) break;
+        step(IC<2>{}, P + 3);
+    }
+}
+
+// ------------------------------------------------------------------ up
+// r, coarse x -> z (prolongation + reverse sweep), with sum z, sum r z, sum r
+// over fluid cells reduced per workgroup into partials.
+template <int CPL, int NW, int RW, typename VCX>
+__global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
+                                                      Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
+                                                      float* __restrict__ z, const int* skip, double* partials) {
+    static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
+    if (skip && *skip) return;
+    __shared__ RowLDS<CPL, NW, RW> L;
+    int ty, kc;
+    const bool active = row_tile(rg, ty, kc) < rg.ntiles;   // idle workgroups still store zero partials
+    RowThread<CPL, NW, RW> th;
+    th.init(g, rg, active ? ty : 0, active ? kc : 0);
+    const int lane = th.lane, w = th.w;
+    for (int e = threadIdx.x; e < 64; e += NW * 64) { L.tab[0][e] = K.rinv[e]; L.tab[1][e] = K.d[e]; }
+    for (int e = threadIdx.x; e < 2 * 2 * NW * 2 * (CPL / 2) * 64; e += NW * 64) (&L.ex[0][0][0][0][0][0])[e] = 0.0f;
+    __syncthreads();
+    double red[3] = {0.0, 0.0, 0.0};
+
+    if (active) {
+        float X[4][RW][CPL], Rq[4][RW][CPL], Iq[4][RW][CPL];
+        float pfr[2][RW][CPL];
+        uint8_t pfn[2][RW][CPL];
+        float pfx[2][RW / 2][CPL / 2];
+        long coff[RW / 2][CPL / 2];
+        bool cok[RW / 2][CPL / 2];
+#pragma unroll
+        for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+            for (int c = 0; c < CPL / 2; c++) {
+                const int ic = (th.i0 + 2 * c + 1) >> 1, jc = (th.jrow[2 * a] + 1) >> 1;
+                cok[a][c] = th.ok[2 * a][2 * c] && ic <= cg.nx && jc >= 1 && jc <= cg.ny;
+                coff[a][c] = (long)ic + (long)jc * cg.sx;
+            }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < RW; t++)
+#pragma unroll
+                for (int q = 0; q < CPL; q++) { X[s][t][q] = 0.0f; Rq[s][t][q] = 0.0f; Iq[s][t][q] = 0.0f; }
+
+        auto issue = [&](int slot, int P) {
+            const bool pin = P >= 1 && P <= g.nz;
+#pragma unroll
+            for (int t = 0; t < RW; t++)
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const bool v = pin && th.ok[t][q];
+                    const long id = th.off[t][q] + (long)P * g.sy;
+                    pfr[slot][t][q] = v ? r[id] : 0.0f;
+                    pfn[slot][t][q] = v ? nbr[id] : (uint8_t)0;
+                }
+            const int kcc = (P + 1) >> 1;
+            const bool cin = pin && kcc >= 1 && kcc <= cg.nz;
+#pragma unroll
+            for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+                for (int c = 0; c < CPL / 2; c++)
+                    pfx[slot][a][c] = (cin && cok[a][c]) ? (float)xc[coff[a][c] + (long)kcc * cg.sy] : 0.0f;
+        };
+        issue(1, th.m0 - 2);
+        issue(0, th.m0 - 1);
+
+        auto step = [&](auto Sc, int P) {
+            constexpr int S = decltype(Sc)::value;
+            constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;
+            constexpr int PB = S & 1, EW = S & 1, ER = EW ^ 1;
+            // S1: plane P: x' = x0 + 2 x_coarse (x0 = r/d on odd cells; the
+            // even x0 is replaced by the reverse sweep, except on isolated cells
+            // where it is 0)
+            sfor<RW>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const uint8_t nb = pfn[PB][t][q];
+                    const bool fl = nb & NB_FLUID;
+                    const float rv = fl ? pfr[PB][t][q] : 0.0f;
+                    const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
+                    const float w2 = fl ? 2.0f * pfx[PB][t / 2][q / 2] : 0.0f;
+                    Rq[S][t][q] = rv;
+                    Iq[S][t][q] = ri;
+                    if constexpr (((q + t + S) & 1) == 1) X[S][t][q] = rv * ri + w2;
+                    else X[S][t][q] = w2;
+                });
+            });
+            issue(PB, P + 2);
+            sfor<CPL>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (((q + S) & 1) == 1) L.ex[EW][0][w][0][q / 2][lane] = X[S][0][q];
+                if constexpr (((q + RW - 1 + S) & 1) == 1) L.ex[EW][0][w][1][q / 2][lane] = X[S][RW - 1][q];
+            });
+            // S2: plane P-1, even cells: z = (r + sum c x'_odd) / d
+            if (P - 1 >= th.m0 - 1) {
+                sfor<RW>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr bool q0 = ((t + S1) & 1) == 0;
+                    const float edge = q0 ? lane_from_left(X[S1][t][CPL - 1]) : lane_from_right(X[S1][t][0]);
+                    sfor<CPL>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        if constexpr (((q + t + S1) & 1) == 0) {
+                            const float s = row_nsum<CPL, NW, RW, t, q, S1, S, S2>(X, edge, L, ER, 0, w, lane, K);
+                            const float ri = Iq[S1][t][q];
+                            X[S1][t][q] = (ri != 0.0f) ? (Rq[S1][t][q] + s) * ri : X[S1][t][q];
+                        }
+                    });
+                });
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if constexpr (((q + S1) & 1) == 0) L.ex[EW][1][w][0][q / 2][lane] = X[S1][0][q];
+                    if constexpr (((q + RW - 1 + S1) & 1) == 0) L.ex[EW][1][w][1][q / 2][lane] = X[S1][RW - 1][q];
+                });
+            }
+            // S3: plane R = P-2, odd cells: z = (r + sum c z_even) / d; output
+            const int R = P - 2;
+            if (R >= th.m0 && R <= th.last) {
+                sfor<RW>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr bool q0 = ((t + S2) & 1) == 1;
+                    const float edge = q0 ? lane_from_left(X[S2][t][CPL - 1]) : lane_from_right(X[S2][t][0]);
+                    sfor<CPL>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        if constexpr (((q + t + S2) & 1) == 1) {
+                            const float s = row_nsum<CPL, NW, RW, t, q, S2, S1, S3>(X, edge, L, ER, 1, w, lane, K);
+                            const float ri = Iq[S2][t][q];
+                            X[S2][t][q] = (ri != 0.0f) ? (Rq[S2][t][q] + s) * ri : X[S2][t][q];
+                        }
+                    });
+                    if (th.tile_row[t]) {
+                        sfor<CPL>([&](auto qc) {
+                            constexpr int q = decltype(qc)::value;
+                            if (th.ok[t][q]) {
+                                const float zv = X[S2][t][q];
+                                z[th.off[t][q] + (long)R * g.sy] = zv;
+                                const double zz = (double)zv, rr = (double)Rq[S2][t][q];
+                                red[0] += zz;
+                                red[1] += rr * zz;
+                                red[2] += rr;
+                            }
+                        });
+                    }
+                });
+            }
+            __syncthreads();
+        };
+        for (int P = th.m0 - 2; P <= th.last + 2; P += 4) {
+            step(IC<3>{}, P);
+            if (P + 1 > th.last + 2) break;
+            step(IC<0>{}, P + 1);
+            if (P + 2 > th.last + 2) break;
+            step(IC<1>{}, P + 2);
+            if (P + 3 > th.last + 2) break;
+            step(IC<2>{}, P + 3);
+        }
+    }
+    block_store<3>(red, partials);
+}
+
+}  // namespace uaamg

@@ -57,6 +57,7 @@ void Engine<D, T, S>::allocate() {
         maxb = std::max(maxb, L.lc.nblocks());
     }
     maxb = std::max(maxb, fused_max_blocks());
+    maxb = std::max(maxb, (long)rows_blocks_);
     partials_ = dalloc<double>(3 * maxb + 64);
     fused_ok_ = fused_supported();
     HIPCHK(hipStreamSynchronize(stream_));
