@@ -88,6 +88,12 @@ def parity_cases():
     pb.solid = balls(pb, [(0.4, 0.1, 0.06)], [0.05])
     out.append(("3d_130x20x12", pb))
     out.append(("2d_40x30_dirichlet", ob.Problem((40, 30), (1.0, 0.75), wall=[ob.DIRICHLET] * 6)))
+    # porous: 40 % random solid cells, many fluid cells with no fluid neighbour
+    pb = ob.Problem((20, 18, 16))
+    m = np.zeros(pb.shape[::-1], np.uint8)
+    m[1:-1, 1:-1, 1:-1] = (rng.random((16, 18, 20)) < 0.4)
+    pb.solid = m.reshape(-1)
+    out.append(("3d_20x18x16_porous", pb))
     return out
 
 

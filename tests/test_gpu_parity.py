@@ -87,7 +87,12 @@ def _demean(p, m):
     return q - q.mean()
 
 
-@pytest.mark.parametrize("name,pb", CASES, ids=[c[0] for c in CASES])
+# PCG needs a connected fluid (one null-space constant): the porous case has
+# isolated fluid cells, so only its hierarchy, matvec and V-cycle are compared.
+PCG_CASES = [c for c in CASES if "porous" not in c[0]]
+
+
+@pytest.mark.parametrize("name,pb", PCG_CASES, ids=[c[0] for c in PCG_CASES])
 def test_pcg_fp64(gpu, name, pb):
     """PCG FP64 vs the oracle at relres 1e-8: iterations within +-1 and
     pressure (mean removed) within 1e-9 of max|p|."""
@@ -104,7 +109,7 @@ def test_pcg_fp64(gpu, name, pb):
     c.close()
 
 
-@pytest.mark.parametrize("name,pb", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("name,pb", PCG_CASES, ids=[c[0] for c in PCG_CASES])
 def test_pcg_fp32(gpu, name, pb):
     """Plain FP32 form (FP32 vectors/levels, FP64 reductions) at relres 1e-6."""
     rng = np.random.default_rng(3)

@@ -16,6 +16,7 @@
 #include "kernels_cg.hpp"
 #include "kernels_march.hpp"
 #include "kernels_rows.hpp"
+#include "kernels_tail.hpp"
 
 namespace uaamg {
 
@@ -161,6 +162,7 @@ public:
         rows_setup(h, wall);
         timer_.s = stream_;
         allocate();
+        tail_setup();
     }
     ~Engine() override { release(); }
 
@@ -276,6 +278,49 @@ public:
         m.grid = dim3(ta, tb, (unsigned)((nm + KC - 1) / KC));
         return m;
     }
+    // ---- tail: the small coarse levels in one workgroup ----
+    static constexpr int TAIL_NLMAX = (D == 3) ? (sizeof(S) == 8 ? 2 : 3) : (sizeof(S) == 8 ? 3 : 5);
+    int tail_first_ = 0;        // first level of the tail (0: no tail)
+    TailArgs<C4> targs_{};
+    void tail_setup() {
+        tail_first_ = 0;
+        if (const char* env = std::getenv("UAAMG_TAIL")) if (env[0] == '0') return;
+        const int nl = nlevels();
+        auto pairs = [&](int l) {
+            const Geo& g = shape_[l];
+            return (long)((g.nx + 1) / 2) * g.ny * (D == 3 ? g.nz : 1);
+        };
+        for (int lt = 1; lt <= nl - 2; lt++) {
+            if (nl - lt > TAIL_NLMAX) continue;
+            if (pairs(lt) > (long)TAIL_MAXP * TAIL_THREADS) continue;
+            bool fit = true;
+            for (int l = lt + 1; l < nl; l++) fit = fit && pairs(l) <= TAIL_THREADS;
+            long elems = 0;
+            for (int l = lt; l < nl; l++) elems += 2 * shape_[l].N;
+            elems += shape_[lt].N;
+            if (!fit || elems * (long)sizeof(SC) > TAIL_LDS_BYTES) continue;
+            tail_first_ = lt;
+            break;
+        }
+        if (!tail_first_) return;
+        targs_.nl = nl - tail_first_;
+        int off = 0;
+        for (int q = 0; q < targs_.nl; q++) {
+            const Level& L = lv_[tail_first_ + q];
+            targs_.g[q] = L.g;
+            targs_.coef[q] = L.coef;
+            targs_.xo[q] = off; off += (int)L.g.N;
+            targs_.bo[q] = off; off += (int)L.g.N;
+        }
+        targs_.ro = off;
+        targs_.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+    }
+    void tail_launch(const int* skip) {
+        targs_.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+        Level& TL = lv_[tail_first_];
+        k_tail<D, S, C4, TAIL_NLMAX><<<1, TAIL_THREADS, 0, stream_>>>(targs_, TL.b, TL.x, skip);
+    }
+
     // ---- level-0 row kernels (3D, FP32 vectors, nx <= 256) ----
     static constexpr int ROWS_NW = 10, ROWS_RW = 2;
     int rows_cpl_ = 0;          // cells per lane (0: row kernels not used)
@@ -348,7 +393,8 @@ public:
     void vcycle_fused(const T* r, T* z, const int* skip) {
         const int nl = nlevels();
         FusedArgs<D, true, CT, C4> A0{g0_, nbr0_, nullptr, c0_};
-        for (int l = 0; l < nl - 1; l++) {
+        const int lt = tail_first_ ? tail_first_ : nl - 1;   // levels >= lt: tail kernel / coarsest solve
+        for (int l = 0; l < lt; l++) {
             Level& C = lv_[l + 1];
             if (l == 0 && rows_cpl_) {
                 TScope ts(timer_, UAAMG_K_DOWN0);
@@ -368,7 +414,10 @@ public:
             }
             HIPCHK(hipGetLastError());
         }
-        {
+        if (tail_first_) {
+            TScope ts(timer_, UAAMG_K_COARSE);
+            tail_launch(skip);
+        } else {
             TScope ts(timer_, UAAMG_K_COARSE);
             Level& C = lv_[nl - 1];
             int npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
@@ -381,7 +430,7 @@ public:
             }
             HIPCHK(hipGetLastError());
         }
-        for (int l = nl - 2; l >= 0; l--) {
+        for (int l = lt - 1; l >= 0; l--) {
             Level& C = lv_[l + 1];
             if (l == 0 && rows_cpl_) {
                 TScope ts(timer_, UAAMG_K_UP0);

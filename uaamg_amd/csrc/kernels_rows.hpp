@@ -74,8 +74,12 @@ __device__ __forceinline__ int row_tile(const RowGeo& rg, int& ty, int& kc) {
 
 __device__ __forceinline__ float rows_rinv(uint8_t nb, const float* srinv, const float* sd, const RowTab& T,
                                            const Geo& g, int i, int j, int k) {
-    if (!(nb & NB_FLUID)) return 0.0f;
-    if (!T.any_dirichlet) return srinv[nb >> 1];
+    const bool fl = nb & NB_FLUID;
+    if (!T.any_dirichlet) {   // uniform branch; the table read itself is unconditional
+        const float ri = srinv[nb >> 1];
+        return fl ? ri : 0.0f;
+    }
+    if (!fl) return 0.0f;
     float d = sd[nb >> 1];
     if (i == 1) d += T.ex[0];
     if (i == g.nx) d += T.ex[1];
@@ -118,14 +122,17 @@ __device__ __forceinline__ void sfor(F&& f) {
     }
 }
 
-// Shared set-up of both kernels: thread geometry and the 4-plane queues.
+// Shared set-up of both kernels.  Addresses are 32-bit element offsets: a
+// row's cells are roff[t] + q.  Rows and planes outside the grid are clamped
+// to the ghost layer (neighbour byte 0: read as non-fluid), lanes past nx are
+// masked with okx.
 template <int CPL, int NW, int RW>
 struct RowThread {
     static constexpr int TY = NW * RW - 4;
     int lane, w, i0, b0, m0, last;
     int jrow[RW];
-    long off[RW][CPL];
-    bool ok[RW][CPL];
+    int roff[RW];
+    bool okx[CPL];
     bool tile_row[RW];
     __device__ void init(const Geo& g, const RowGeo& rg, int ty, int kc) {
         lane = threadIdx.x & 63;
@@ -135,15 +142,18 @@ struct RowThread {
         m0 = 1 + kc * rg.KC;
         last = min(m0 + rg.KC - 1, g.nz);
 #pragma unroll
+        for (int q = 0; q < CPL; q++) okx[q] = i0 + q <= g.nx;
+#pragma unroll
         for (int t = 0; t < RW; t++) {
             jrow[t] = b0 - 2 + w * RW + t;
             tile_row[t] = (w * RW + t >= 2) && (w * RW + t < TY + 2) && jrow[t] <= g.ny;
-#pragma unroll
-            for (int q = 0; q < CPL; q++) {
-                ok[t][q] = (i0 + q <= g.nx) && jrow[t] >= 1 && jrow[t] <= g.ny;
-                off[t][q] = (long)(i0 + q) + (long)jrow[t] * g.sx;
-            }
+            const int jc = min(max(jrow[t], 0), g.ny + 1);
+            roff[t] = min(i0, g.nx + 1) + jc * (int)g.sx;
         }
+    }
+    // element offset of plane P (clamped to the ghost planes)
+    __device__ __forceinline__ int plane(const Geo& g, int P) const {
+        return min(max(P, 0), g.nz + 1) * (int)g.sy;
     }
 };
 
@@ -166,7 +176,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
     for (int e = threadIdx.x; e < 2 * 2 * NW * 2 * (CPL / 2) * 64; e += NW * 64) (&L.ex[0][0][0][0][0][0])[e] = 0.0f;
     __syncthreads();
 
-    float X[4][RW][CPL], Rq[4][RW][CPL], Iq[4][RW][CPL];
+    float X[4][RW][CPL], Rq[2][RW][CPL], Iq[4][RW][CPL];
     float pfr[2][RW][CPL];
     uint8_t pfn[2][RW][CPL];
     float acc[RW / 2][CPL / 2];
@@ -175,22 +185,25 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
 #pragma unroll
         for (int t = 0; t < RW; t++)
 #pragma unroll
-            for (int q = 0; q < CPL; q++) { X[s][t][q] = 0.0f; Rq[s][t][q] = 0.0f; Iq[s][t][q] = 0.0f; }
+            for (int q = 0; q < CPL; q++) { X[s][t][q] = 0.0f; Rq[s & 1][t][q] = 0.0f; Iq[s][t][q] = 0.0f; }
 #pragma unroll
     for (int a = 0; a < RW / 2; a++)
 #pragma unroll
         for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
 
     auto issue = [&](int slot, int P) {
-        const bool pin = P >= 1 && P <= g.nz;
+        // unconditional loads from clamped addresses: straight-line code lets
+        // the compiler wait for exactly the loads a step consumes
+        const int po = th.plane(g, P);
 #pragma unroll
         for (int t = 0; t < RW; t++)
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
-                const bool v = pin && th.ok[t][q];
-                const long id = th.off[t][q] + (long)P * g.sy;
-                pfr[slot][t][q] = v ? r[id] : 0.0f;
-                pfn[slot][t][q] = v ? nbr[id] : (uint8_t)0;
+                const int id = th.roff[t] + po + q;
+                const float rv = r[id];
+                const uint8_t nv = nbr[id];
+                pfr[slot][t][q] = th.okx[q] ? rv : 0.0f;
+                pfn[slot][t][q] = th.okx[q] ? nv : (uint8_t)0;
             }
     };
     issue(1, th.m0 - 2);
@@ -212,10 +225,12 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
                 const uint8_t nb = pfn[PB][t][q];
                 const float rv = (nb & NB_FLUID) ? pfr[PB][t][q] : 0.0f;
                 const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
-                Rq[S][t][q] = rv;
+                Rq[S & 1][t][q] = rv;
                 Iq[S][t][q] = ri;
-                if constexpr (((q + t + S) & 1) == 1) X[S][t][q] = rv * ri;
-                else X[S][t][q] = 0.0f;
+                // isolated fluid cells (ri == 0, no fluid neighbour: their x is
+                // never read) carry r, their residual; non-fluid cells carry 0
+                if constexpr (((q + t + S) & 1) == 1) X[S][t][q] = (ri != 0.0f) ? rv * ri : rv;
+                else X[S][t][q] = (ri != 0.0f) ? 0.0f : rv;
             });
         });
         issue(PB, P + 2);
@@ -234,7 +249,10 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
                     constexpr int q = decltype(qc)::value;
                     if constexpr (((q + t + S1) & 1) == 0) {
                         const float s = row_nsum<CPL, NW, RW, t, q, S1, S, S2>(X, edge, L, ER, 0, w, lane, K);
-                        X[S1][t][q] = (Rq[S1][t][q] + s) * Iq[S1][t][q];
+                        // (r + s) / d on cells with 1/d != 0; isolated and non-fluid
+                        // cells keep X.  Written as a blend so it stays branch-free.
+                        const float ri = Iq[S1][t][q];
+                        X[S1][t][q] = (Rq[S1 & 1][t][q] + s) * ri + ((ri != 0.0f) ? 0.0f : X[S1][t][q]);
                     }
                 });
             });
@@ -262,8 +280,11 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
                     constexpr int q = decltype(qc)::value;
                     if constexpr (((q + t + S2) & 1) == 1) {
                         const float s = row_nsum<CPL, NW, RW, t, q, S2, S1, S3>(X, edge, L, ER, 1, w, lane, K);
-                        const float res = (Iq[S2][t][q] != 0.0f) ? s : Rq[S2][t][q];
-                        acc[t / 2][q / 2] += res;
+                        const bool iso = Iq[S2][t][q] == 0.0f;
+                        acc[t / 2][q / 2] += (iso ? 0.0f : s) + (iso ? X[S2][t][q] : 0.0f);
+                    } else {
+                        // even cells: residual 0 after the even update, r on isolated cells
+                        acc[t / 2][q / 2] += (Iq[S2][t][q] != 0.0f) ? 0.0f : X[S2][t][q];
                     }
                 });
             });
@@ -277,7 +298,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
                     for (int c = 0; c < CPL / 2; c++) {
                         const int ic = (th.i0 + 2 * c + 1) >> 1;
                         if (ic <= cg.nx && jc <= cg.ny && kcc <= cg.nz)
-                            bc[(long)ic + (long)jc * cg.sx + (long)kcc * cg.sy] = (VC)acc[a][c];
+                            bc[ic + jc * (int)cg.sx + kcc * (int)cg.sy] = (VC)acc[a][c];
                     }
                 }
             }
@@ -320,15 +341,16 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
         float pfr[2][RW][CPL];
         uint8_t pfn[2][RW][CPL];
         float pfx[2][RW / 2][CPL / 2];
-        long coff[RW / 2][CPL / 2];
+        bool pcin[2];
+        int coff[RW / 2][CPL / 2];
         bool cok[RW / 2][CPL / 2];
 #pragma unroll
         for (int a = 0; a < RW / 2; a++)
 #pragma unroll
             for (int c = 0; c < CPL / 2; c++) {
                 const int ic = (th.i0 + 2 * c + 1) >> 1, jc = (th.jrow[2 * a] + 1) >> 1;
-                cok[a][c] = th.ok[2 * a][2 * c] && ic <= cg.nx && jc >= 1 && jc <= cg.ny;
-                coff[a][c] = (long)ic + (long)jc * cg.sx;
+                cok[a][c] = th.okx[2 * c] && th.jrow[2 * a] >= 1 && th.jrow[2 * a] <= g.ny && ic <= cg.nx && jc >= 1 && jc <= cg.ny;
+                coff[a][c] = min(ic, cg.nx + 1) + min(max(jc, 0), cg.ny + 1) * (int)cg.sx;
             }
 #pragma unroll
         for (int s = 0; s < 4; s++)
@@ -338,23 +360,26 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
                 for (int q = 0; q < CPL; q++) { X[s][t][q] = 0.0f; Rq[s][t][q] = 0.0f; Iq[s][t][q] = 0.0f; }
 
         auto issue = [&](int slot, int P) {
-            const bool pin = P >= 1 && P <= g.nz;
+            const int po = th.plane(g, P);
 #pragma unroll
             for (int t = 0; t < RW; t++)
 #pragma unroll
                 for (int q = 0; q < CPL; q++) {
-                    const bool v = pin && th.ok[t][q];
-                    const long id = th.off[t][q] + (long)P * g.sy;
-                    pfr[slot][t][q] = v ? r[id] : 0.0f;
-                    pfn[slot][t][q] = v ? nbr[id] : (uint8_t)0;
+                    const int id = th.roff[t] + po + q;
+                    const float rv = r[id];
+                    const uint8_t nv = nbr[id];
+                    pfr[slot][t][q] = th.okx[q] ? rv : 0.0f;
+                    pfn[slot][t][q] = th.okx[q] ? nv : (uint8_t)0;
                 }
+            // coarse x of the parents; masked where it is used (S1), so the
+            // loads stay in flight
             const int kcc = (P + 1) >> 1;
-            const bool cin = pin && kcc >= 1 && kcc <= cg.nz;
+            pcin[slot] = P >= 1 && P <= g.nz && kcc >= 1 && kcc <= cg.nz;
+            const int cpo = min(max(kcc, 0), cg.nz + 1) * (int)cg.sy;
 #pragma unroll
             for (int a = 0; a < RW / 2; a++)
 #pragma unroll
-                for (int c = 0; c < CPL / 2; c++)
-                    pfx[slot][a][c] = (cin && cok[a][c]) ? (float)xc[coff[a][c] + (long)kcc * cg.sy] : 0.0f;
+                for (int c = 0; c < CPL / 2; c++) pfx[slot][a][c] = (float)xc[coff[a][c] + cpo];
         };
         issue(1, th.m0 - 2);
         issue(0, th.m0 - 1);
@@ -374,7 +399,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
                     const bool fl = nb & NB_FLUID;
                     const float rv = fl ? pfr[PB][t][q] : 0.0f;
                     const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
-                    const float w2 = fl ? 2.0f * pfx[PB][t / 2][q / 2] : 0.0f;
+                    const float w2 = (fl && pcin[PB] && cok[t / 2][q / 2]) ? 2.0f * pfx[PB][t / 2][q / 2] : 0.0f;
                     Rq[S][t][q] = rv;
                     Iq[S][t][q] = ri;
                     if constexpr (((q + t + S) & 1) == 1) X[S][t][q] = rv * ri + w2;
@@ -398,7 +423,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
                         if constexpr (((q + t + S1) & 1) == 0) {
                             const float s = row_nsum<CPL, NW, RW, t, q, S1, S, S2>(X, edge, L, ER, 0, w, lane, K);
                             const float ri = Iq[S1][t][q];
-                            X[S1][t][q] = (ri != 0.0f) ? (Rq[S1][t][q] + s) * ri : X[S1][t][q];
+                            X[S1][t][q] = (Rq[S1][t][q] + s) * ri + ((ri != 0.0f) ? 0.0f : X[S1][t][q]);
                         }
                     });
                 });
@@ -420,15 +445,15 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
                         if constexpr (((q + t + S2) & 1) == 1) {
                             const float s = row_nsum<CPL, NW, RW, t, q, S2, S1, S3>(X, edge, L, ER, 1, w, lane, K);
                             const float ri = Iq[S2][t][q];
-                            X[S2][t][q] = (ri != 0.0f) ? (Rq[S2][t][q] + s) * ri : X[S2][t][q];
+                            X[S2][t][q] = (Rq[S2][t][q] + s) * ri + ((ri != 0.0f) ? 0.0f : X[S2][t][q]);
                         }
                     });
                     if (th.tile_row[t]) {
                         sfor<CPL>([&](auto qc) {
                             constexpr int q = decltype(qc)::value;
-                            if (th.ok[t][q]) {
+                            if (th.okx[q]) {
                                 const float zv = X[S2][t][q];
-                                z[th.off[t][q] + (long)R * g.sy] = zv;
+                                z[th.roff[t] + q + R * (int)g.sy] = zv;
                                 const double zz = (double)zv, rr = (double)Rq[S2][t][q];
                                 red[0] += zz;
                                 red[1] += rr * zz;
