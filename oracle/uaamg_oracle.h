@@ -11,7 +11,7 @@
  *
  * Every function follows the reference loop and summation order so that, in
  * FP64 and with the default options, results match the compiled reference
- * bit for bit (checked against oracle/_ref in tests/test_oracle_ref.py).
+ * bit for bit (checked against oracle/_ref in tests/test_oracle.py).
  */
 #ifndef UAAMG_ORACLE_H
 #define UAAMG_ORACLE_H

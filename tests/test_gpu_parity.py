@@ -137,3 +137,39 @@ def test_amgpcg(gpu, N, prec):
     tol = 1e-10 if prec == U.FP64 else 1e-4
     assert abs(it - io) <= (0 if prec == U.FP64 else 1), (it, io)
     assert np.abs(x - xo).max() <= tol * np.abs(xo).max()
+
+
+# ---- mixed form: BF16 level storage, FP32 CG, FP64 reductions ----
+
+@pytest.mark.parametrize("name,pb", CASES, ids=[c[0] for c in CASES])
+def test_vcycle_bf16_levels(gpu, name, pb):
+    """BF16 coarse levels (8-bit mantissa couplings, b, x) perturb the
+    preconditioner at the 1e-2 level: it stays close to the FP64 V-cycle."""
+    rng = np.random.default_rng(11)
+    r = cases.random_rhs(pb, rng)
+    c = ctx_for(pb, precision=U.MIXED_BF16)
+    z = c.apply(r)
+    zo = ob.uaamg_apply(pb, r)
+    m = np.zeros(pb.N, bool); pb.interior(m)[...] = True
+    err = np.abs(z[m] - zo[m]).max() / np.abs(zo[m]).max()
+    assert err <= 5e-2, err
+    c.close()
+
+
+@pytest.mark.parametrize("flexible", [0, 1])
+@pytest.mark.parametrize("name,pb", PCG_CASES, ids=[c[0] for c in PCG_CASES])
+def test_pcg_bf16_levels(gpu, name, pb, flexible):
+    """The CG residual, not the preconditioner, sets the accuracy: at relres
+    1e-6 the pressure is within 1e-5 of the FP64 oracle (mean removed);
+    the iteration count may grow by the preconditioner's perturbation."""
+    rng = np.random.default_rng(3)
+    rhs = 1e3 * cases.random_rhs(pb, rng)
+    po, res = ob.pcg(pb, rhs, 500, 1e-10, tol_mode=1)
+    c = ctx_for(pb, precision=U.MIXED_BF16, tol_mode=U.TOL_REL, flexible=flexible)
+    p, st = c.solve(rhs, 500, 1e-6)
+    assert st.status == 0, U.status_name(st.status)
+    assert st.iters <= 1.5 * res.iters + 5, (st.iters, res.iters)
+    m = pb.fluid_interior_mask()
+    a, b = _demean(p, m), _demean(po, m)
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    c.close()

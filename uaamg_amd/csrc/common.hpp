@@ -61,9 +61,15 @@ enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_BREAKDOWN = 3, ST_NO
 template <typename S> struct CTOf { using type = float; };
 template <> struct CTOf<double> { using type = double; };
 
+// Level record {cx, cy, cz, diag} of a storage type; 16-bit levels use an
+// 8-byte record (one load per cell).
+struct alignas(8) BF16x4 { __bf16 x, y, z, w; };
+struct alignas(8) F16x4 { _Float16 x, y, z, w; };
 template <typename S> struct Vec4;
 template <> struct Vec4<double> { using type = double4; };
 template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<__bf16> { using type = BF16x4; };
+template <> struct Vec4<_Float16> { using type = F16x4; };
 
 // linear block id / count for 3D grids
 __device__ __forceinline__ int lin_block() {

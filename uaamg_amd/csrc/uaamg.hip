@@ -284,13 +284,15 @@ static std::unique_ptr<EngineBase> make_engine(const uaamg_opts& o, const uaamg_
         switch (o.precision) {
             case UAAMG_FP64: return std::make_unique<Engine<2, double, double>>(o, n, h, g.wall, s);
             case UAAMG_FP32: return std::make_unique<Engine<2, float, float>>(o, n, h, g.wall, s);
-            default: throw HipError{hipErrorInvalidValue, "mixed precision not built"};
+            case UAAMG_MIXED_BF16: return std::make_unique<Engine<2, float, __bf16>>(o, n, h, g.wall, s);
+            default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
         }
     }
     switch (o.precision) {
         case UAAMG_FP64: return std::make_unique<Engine<3, double, double>>(o, n, h, g.wall, s);
         case UAAMG_FP32: return std::make_unique<Engine<3, float, float>>(o, n, h, g.wall, s);
-        default: throw HipError{hipErrorInvalidValue, "mixed precision not built"};
+        case UAAMG_MIXED_BF16: return std::make_unique<Engine<3, float, __bf16>>(o, n, h, g.wall, s);
+        default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
     }
 }
 
