@@ -105,6 +105,112 @@ def load_traffic(workload):
         return None
 
 
+def bench_batch(args):
+    """Batched mixed-size stream: 64 2D grids (side log-uniform 64..1024) and
+    32 3D grids (side 32..160), random masks and walls, relres 1e-6.  The
+    plain form solves them one after another with single-grid FP32 contexts
+    (eager and hipGraph); the batched form runs all 96 in one set of launches
+    (FP32, and BF16 levels with FP32 CG and FP64 reductions; hipGraph replay,
+    autotuned launch shapes).  Hierarchies are resident in every form (built
+    before the timed region).  Errors are measured against the FP64 GPU path,
+    which matches the FP64 oracle to 1e-9 of max|p| (parity tests)."""
+    import torch
+    import uaamg_amd as U
+    from uaamg_amd import workloads as W
+    t0 = time.perf_counter()
+    pbs = W.batch_mixed(seed=0)
+    log(f"[batch] generated {len(pbs)} grids ({sum(p.cells for p in pbs) / 1e6:.1f} M cells) in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    rhs = [torch.from_numpy(p.rhs).to(dev) for p in pbs]
+    tol = 1e-6
+
+    def plain(prec, use_graph, reps):
+        ctxs = []
+        for p in pbs:
+            c = U.Context(precision=prec, tol_mode=U.TOL_REL, use_graph=use_graph, max_iter=1000)
+            c.set_grid(p.n, p.h, p.solid, p.wall)
+            ctxs.append(c)
+        outs = [torch.zeros(p.N, dtype=torch.float64, device=dev) for p in pbs]
+        for k, c in enumerate(ctxs):      # warm-up (and graph capture)
+            c.solve_device(rhs[k].data_ptr(), outs[k].data_ptr(), 0, 1000, tol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        iters = []
+        for _ in range(reps):
+            for k, c in enumerate(ctxs):
+                outs[k].zero_()
+                st = c.solve_device(rhs[k].data_ptr(), outs[k].data_ptr(), 0, 1000, tol)
+                iters.append(st.iters)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        for c in ctxs:
+            c.close()
+        return dt, outs, iters
+
+    def batched(prec, reps):
+        b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000)
+        for p in pbs:
+            b.add(p.n, p.h, p.solid, p.wall)
+        outs = [torch.zeros(p.N, dtype=torch.float64, device=dev) for p in pbs]
+        R = [t.data_ptr() for t in rhs]
+        P = [t.data_ptr() for t in outs]
+        b.solve_device(R, P, None, 1000, tol)      # warm-up: autotuner + graph capture
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            for o in outs:
+                o.zero_()
+            rc, st = b.solve_device(R, P, None, 1000, tol)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        b.close()
+        return dt, outs, [s.iters for s in st], rc
+
+    reps = 2
+    t_plain, out_plain, it_plain = plain(U.FP32, 0, reps)
+    log(f"[batch] plain fp32 eager: {t_plain * 1e3:.1f} ms for {len(pbs)} solves")
+    t_plain_g, _, _ = plain(U.FP32, 1, reps)
+    log(f"[batch] plain fp32 graph: {t_plain_g * 1e3:.1f} ms")
+    t_b32, out_b32, it_b32, rc32 = batched(U.FP32, reps)
+    log(f"[batch] batched fp32: {t_b32 * 1e3:.1f} ms")
+    t_bbf, out_bbf, it_bbf, rcbf = batched(U.MIXED_BF16, reps)
+    log(f"[batch] batched bf16 levels: {t_bbf * 1e3:.1f} ms")
+    _, out_64, _ = plain(U.FP64, 0, 1)
+
+    def err(outs):
+        e = 0.0
+        for p, a, b in zip(pbs, outs, out_64):
+            m = torch.from_numpy(p.fluid_mask()).to(dev)
+            x, y = a[m], b[m]
+            if all(w == U.NEUMANN for w in p.wall[:2 * p.dim]):
+                x = x - x.mean()
+                y = y - y.mean()
+            e = max(e, float((x - y).abs().max() / y.abs().max().clamp_min(1e-300)))
+        return e
+
+    n = len(pbs)
+    best_plain = min(t_plain, t_plain_g)
+    res = {
+        "workload": "batch_mixed: 64 2D grids side logU[64,1024] + 32 3D grids side logU[32,160], random masks "
+                    "(0-30 % boxes/spheres), random Neumann/Dirichlet walls, random-vortex rhs (seed 0), relres 1e-6",
+        "members": n,
+        "cells": int(sum(p.cells for p in pbs)),
+        "plain_fp32_solves_per_s": round(n / t_plain, 2),
+        "plain_fp32_graph_solves_per_s": round(n / t_plain_g, 2),
+        "batched_fp32_solves_per_s": round(n / t_b32, 2),
+        "batched_bf16_solves_per_s": round(n / t_bbf, 2),
+        "speedup_bf16_batched_vs_best_plain": round(best_plain / t_bbf, 3),
+        "max_rel_err_vs_fp64": {"plain_fp32": err(out_plain), "batched_fp32": err(out_b32),
+                                "batched_bf16": err(out_bbf)},
+        "iters_max": {"plain_fp32": int(max(it_plain)), "batched_fp32": int(max(it_b32)),
+                      "batched_bf16": int(max(it_bbf))},
+        "status": {"batched_fp32": U.status_name(rc32), "batched_bf16": U.status_name(rcbf)},
+        "hierarchies": "resident (built before timing)",
+    }
+    log(f"[batch] {json.dumps(res)}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +220,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-size stream")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched mixed-size stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (3x), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -123,6 +231,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent solves shard per GPU with no communication; a single solve would need z-slab halos.", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        print(json.dumps({"batched": bench_batch(args)}), flush=True)
+        return
 
     import torch
     import uaamg_amd as U
@@ -242,6 +354,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        del rhs_d, solid_d, p_d
+        ctx.close()
+        out["batched"] = bench_batch(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(pb, k_med)

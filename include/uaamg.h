@@ -152,6 +152,36 @@ int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, double tol,
 /* Device memory (bytes) a context would hold for grid g under opts o. */
 size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o);
 
+/* ---- batches: many independent solves in one set of launches ----
+ * A batch holds grids of any size, mask and walls (2D and 3D).  Each member
+ * is solved with the semantics of uaamg_solve_device (one Solver::solve per
+ * member, include/solver/solver.h:29-30 / pcg_3d.cpp:18-125); the batch runs
+ * every member's PCG iteration as one fixed sequence of launches over a
+ * (grid, level, tile) work-list, captured as a hipGraph when opts.use_graph,
+ * with launch shapes picked by a built-in autotuner on the first solve.
+ * precision: UAAMG_FP32 or UAAMG_MIXED_BF16; V(1,1) (nu_pre = nu_post = 1),
+ * fused = 1.  One host thread per batch. */
+typedef struct uaamg_batch uaamg_batch;
+int uaamg_batch_create(uaamg_batch** out, const uaamg_opts* o);
+void uaamg_batch_destroy(uaamg_batch* b);
+const char* uaamg_batch_last_error(const uaamg_batch* b);
+/* Add a member and build its hierarchy on device (as uaamg_set_grid);
+ * *id = member index (order of addition). g->solid is a host pointer. */
+int uaamg_batch_add(uaamg_batch* b, const uaamg_grid* g, int* id);
+/* Same, g->solid is a device pointer. */
+int uaamg_batch_add_device(uaamg_batch* b, const uaamg_grid* g, int* id);
+/* Remove every member (device memory is released). */
+int uaamg_batch_clear(uaamg_batch* b);
+int uaamg_batch_size(const uaamg_batch* b);
+size_t uaamg_batch_bytes(const uaamg_batch* b);
+/* Solve every member: rhs_dev[i], p_dev[i] are padded FP64 device arrays of
+ * member i; warm_start[i] as in uaamg_solve_device (NULL: all cold).  st
+ * (NULL or an array of uaamg_batch_size entries) receives per-member stats;
+ * ms_total is the wall time of the whole call.  Returns UAAMG_OK or the
+ * first member's non-OK status. */
+int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, double* const* p_dev,
+                      const int* warm_start, int max_iter, double tol, uaamg_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

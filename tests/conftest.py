@@ -13,24 +13,18 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: larger sizes")
 
 
-def _gpu_present():
-    try:
-        import uaamg_amd
-        lib = uaamg_amd.load()
-        import ctypes
-        h = ctypes.c_void_p()
-        rc = lib.L.uaamg_create(ctypes.byref(h), None)
-        if rc == 0:
-            lib.L.uaamg_destroy(h)
-            return True
-        return False
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def gpu():
-    """The loaded HIP library; GPU tests fail loudly when it is missing or no device is usable."""
+    """The loaded HIP library; GPU tests fail loudly when it is missing or no device is usable.
+
+    torch (used by some tests for device buffers) brings its own HIP runtime
+    library; it is initialised first so that the HIP library here binds to
+    the same runtime instead of loading a second one."""
+    try:
+        import torch
+        torch.cuda.init()
+    except ImportError:
+        pass
     import uaamg_amd
     lib = uaamg_amd.load()
     import ctypes

@@ -160,8 +160,9 @@ def test_vcycle_bf16_levels(gpu, name, pb):
 @pytest.mark.parametrize("name,pb", PCG_CASES, ids=[c[0] for c in PCG_CASES])
 def test_pcg_bf16_levels(gpu, name, pb, flexible):
     """The CG residual, not the preconditioner, sets the accuracy: at relres
-    1e-6 the pressure is within 1e-5 of the FP64 oracle (mean removed);
-    the iteration count may grow by the preconditioner's perturbation."""
+    1e-6 the pressure is within 2e-5 of the FP64 oracle (mean removed), the
+    bar of the plain FP32 form; the iteration count may grow by the
+    preconditioner's perturbation."""
     rng = np.random.default_rng(3)
     rhs = 1e3 * cases.random_rhs(pb, rng)
     po, res = ob.pcg(pb, rhs, 500, 1e-10, tol_mode=1)
@@ -171,5 +172,28 @@ def test_pcg_bf16_levels(gpu, name, pb, flexible):
     assert st.iters <= 1.5 * res.iters + 5, (st.iters, res.iters)
     m = pb.fluid_interior_mask()
     a, b = _demean(p, m), _demean(po, m)
-    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max()
     c.close()
+
+
+# ---- hipGraph replay of the CG iterations ----
+
+@pytest.mark.parametrize("prec", [U.FP64, U.FP32, U.MIXED_BF16])
+@pytest.mark.parametrize("name,pb", [c for c in PCG_CASES if c[0] in ("3d_24x20x18_aniso_solids", "2d_48x40_aniso_solids", "3d_40x36x34_walls_solids")],
+                         ids=lambda v: v if isinstance(v, str) else "")
+def test_graph_replay_matches_eager(gpu, name, pb, prec):
+    """use_graph=1 replays captured 4-iteration chunks: every solve (several
+    replays each, with eager work and synchronisation in between) gives the
+    eager result bit for bit."""
+    rng = np.random.default_rng(5)
+    rhs = 1e3 * cases.random_rhs(pb, rng)
+    eager = ctx_for(pb, precision=prec, tol_mode=U.TOL_REL, use_graph=0)
+    p0, st0 = eager.solve(rhs, 400, 1e-6)
+    graph = ctx_for(pb, precision=prec, tol_mode=U.TOL_REL, use_graph=1)
+    for rep in range(3):
+        p1, st1 = graph.solve(rhs, 400, 1e-6)
+        assert st1.status == st0.status and st1.iters == st0.iters, (rep, st1.iters, st0.iters)
+        assert np.array_equal(p1, p0), rep
+        pe, _ = eager.solve(rhs, 400, 1e-6)       # eager work between the replays
+        assert np.array_equal(pe, p0)
+    eager.close(); graph.close()

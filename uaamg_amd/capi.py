@@ -42,7 +42,10 @@ class Stats(C.Structure):
 EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",
            "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_solve", "uaamg_solve_device",
            "uaamg_apply_precond", "uaamg_matvec", "uaamg_hierarchy", "uaamg_level_data",
-           "uaamg_amgpcg", "uaamg_bytes_required"]
+           "uaamg_amgpcg", "uaamg_bytes_required",
+           "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
+           "uaamg_batch_add_device", "uaamg_batch_clear", "uaamg_batch_size", "uaamg_batch_bytes",
+           "uaamg_batch_solve"]
 
 
 def lib_path():
@@ -75,6 +78,18 @@ class Lib:
         L.uaamg_amgpcg.argtypes = [I, P, P, I, D, P, I]
         L.uaamg_bytes_required.argtypes = [P, P]
         L.uaamg_bytes_required.restype = C.c_size_t
+        L.uaamg_batch_create.argtypes = [C.POINTER(C.c_void_p), P]
+        L.uaamg_batch_destroy.argtypes = [P]
+        L.uaamg_batch_destroy.restype = None
+        L.uaamg_batch_last_error.argtypes = [P]
+        L.uaamg_batch_last_error.restype = C.c_char_p
+        L.uaamg_batch_add.argtypes = [P, P, P]
+        L.uaamg_batch_add_device.argtypes = [P, P, P]
+        L.uaamg_batch_clear.argtypes = [P]
+        L.uaamg_batch_size.argtypes = [P]
+        L.uaamg_batch_bytes.argtypes = [P]
+        L.uaamg_batch_bytes.restype = C.c_size_t
+        L.uaamg_batch_solve.argtypes = [P, P, P, P, I, D, P]
 
     def default_opts(self, **kw):
         o = Opts()
@@ -218,3 +233,74 @@ class Context:
         out["solid"] = solid
         out["dims"] = dims
         return out
+
+
+class Batch:
+    """Many independent solves run as one set of launches (uaamg_batch_*)."""
+
+    def __init__(self, lib=None, **opts):
+        self.lib = lib or load()
+        opts.setdefault("precision", FP32)
+        self.opts = self.lib.default_opts(**opts)
+        h = C.c_void_p()
+        rc = self.lib.L.uaamg_batch_create(C.byref(h), C.byref(self.opts))
+        if rc != OK:
+            raise RuntimeError(f"uaamg_batch_create failed: {status_name(rc)}")
+        self.h = h
+        self.members = []
+
+    h = None
+
+    def close(self):
+        if self.h:
+            self.lib.L.uaamg_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _err(self, what, rc):
+        msg = self.lib.L.uaamg_batch_last_error(self.h).decode()
+        raise RuntimeError(f"{what}: {status_name(rc)}: {msg}")
+
+    def add(self, n, h, solid, wall=None, device=False):
+        """Add a member; solid is a host numpy mask or (device=True) a device pointer."""
+        if device:
+            g = make_grid(n, h, np.zeros(1, np.uint8), wall)
+            g.solid = C.c_void_p(int(solid))
+            fn = self.lib.L.uaamg_batch_add_device
+        else:
+            solid = np.ascontiguousarray(solid, dtype=np.uint8)
+            g = make_grid(n, h, solid, wall)
+            fn = self.lib.L.uaamg_batch_add
+        i = C.c_int(-1)
+        rc = fn(self.h, C.byref(g), C.byref(i))
+        if rc != OK:
+            self._err("uaamg_batch_add", rc)
+        self.members.append(tuple(n))
+        return i.value
+
+    def clear(self):
+        rc = self.lib.L.uaamg_batch_clear(self.h)
+        if rc != OK:
+            self._err("uaamg_batch_clear", rc)
+        self.members = []
+
+    def size(self):
+        return self.lib.L.uaamg_batch_size(self.h)
+
+    def bytes(self):
+        return self.lib.L.uaamg_batch_bytes(self.h)
+
+    def solve_device(self, rhs_ptrs, p_ptrs, warm=None, max_iter=-1, tol=-1.0):
+        """rhs_ptrs / p_ptrs: device pointers (ints) of every member, in order.
+        Returns (status, [Stats])."""
+        n = len(rhs_ptrs)
+        assert n == len(p_ptrs) == self.size()
+        R = (C.c_void_p * n)(*[C.c_void_p(int(x)) for x in rhs_ptrs])
+        Pp = (C.c_void_p * n)(*[C.c_void_p(int(x)) for x in p_ptrs])
+        W = (C.c_int * n)(*([int(w) for w in warm] if warm is not None else [0] * n))
+        st = (Stats * n)()
+        rc = self.lib.L.uaamg_batch_solve(self.h, R, Pp, W, max_iter, tol, st)
+        if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
+            self._err("uaamg_batch_solve", rc)
+        return rc, list(st)

@@ -125,18 +125,22 @@ __device__ __forceinline__ void block_store(double (&v)[NV], double* partials) {
     }
 }
 
-// In a single-workgroup kernel: totals of the partials of a producer launch
-// of nb blocks; valid in thread 0.
+// In one workgroup: totals of the partials of blocks [base, base + nb) of a
+// producer launch of nbt blocks; valid in thread 0.
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&tot)[NV]) {
+__device__ __forceinline__ void reduce_partials_at(const double* partials, int nbt, int base, int nb, double (&tot)[NV]) {
     int t = lin_thread(), nt = n_threads();
 #pragma unroll
     for (int q = 0; q < NV; q++) {
         double s = 0.0;
-        for (int x = t; x < nb; x += nt) s += partials[(long)q * nb + x];
+        for (int x = t; x < nb; x += nt) s += partials[(long)q * nbt + base + x];
         tot[q] = s;
     }
     block_sum<NV>(tot);
+}
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* partials, int nb, double (&tot)[NV]) {
+    reduce_partials_at<NV>(partials, nb, 0, nb, tot);
 }
 
 }  // namespace uaamg

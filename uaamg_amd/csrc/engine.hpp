@@ -17,6 +17,7 @@
 #include "kernels_march.hpp"
 #include "kernels_rows.hpp"
 #include "kernels_tail.hpp"
+#include "kernels_batch.hpp"
 
 namespace uaamg {
 
@@ -164,7 +165,7 @@ public:
         allocate();
         tail_setup();
     }
-    ~Engine() override { release(); }
+    ~Engine() override { destroy_graph(); release(); }
 
     void set_opts(const uaamg_opts& o) override {
         int all_neumann = 1;
@@ -174,6 +175,7 @@ public:
         o_.check_every = o.check_every; o_.use_graph = o.use_graph; o_.timing = o.timing; o_.flexible = o.flexible;
         o_.project_mean = o.project_mean;
         project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
+        destroy_graph();
     }
     void set_stream(hipStream_t s) override { stream_ = s; timer_.s = s; }
     int nlevels() const override { return (int)shape_.size(); }
@@ -202,7 +204,7 @@ public:
             k_level_diag<D, C4, S><<<C.lc.grid, C.lc.block, 0, stream_>>>(C.coef, C.ex, C.mask, C.g, C.lc.kz);
             HIPCHK(hipGetLastError());
         }
-        graph_valid_ = false;
+        destroy_graph();
     }
 
     // ---------------- V-cycle ----------------
@@ -278,6 +280,41 @@ public:
         m.grid = dim3(ta, tb, (unsigned)((nm + KC - 1) / KC));
         return m;
     }
+    // ---- batch membership: this grid's buffers and launch shapes ----
+    bool batch_ready() const {
+        return std::is_same<T, float>::value && fused_ok_ && tail_first_ > 0 && o_.nu_pre == 1 && o_.nu_post == 1
+            && nlevels() <= BATCH_MAXL;
+    }
+    double alg_bytes_of(int iters) const { return alg_bytes(iters); }
+    long padded_cells() const { return g0_.N; }
+    long n4() const { return N4_; }
+    int rows_cpl() const { return rows_cpl_; }
+    int tail_first() const { return tail_first_; }
+    // fill the batch descriptor (ctrl: the batch's scalar slot for this grid)
+    void fill_batch(BGrid<S>& d, Ctrl* ctrl) const {
+        if constexpr (std::is_same<T, float>::value) {
+            d.g0 = g0_; d.N4 = N4_;
+            d.r = r_; d.z = z_; d.p = p_; d.Ap = Ap_; d.x = x_;
+            d.nbr0 = nbr0_;
+            d.c0d = c0d_;
+            d.c0 = c0_;
+            d.ctrl = ctrl;
+            d.count = count_;
+            d.nl = nlevels();
+            d.lt = tail_first_;
+            for (int l = 0; l < d.nl && l < BATCH_MAXL; l++) {
+                d.lg[l] = lv_[l].g;
+                d.lb[l] = lv_[l].b; d.lx[l] = lv_[l].x; d.lcoef[l] = lv_[l].coef;
+                d.lKC[l] = 8;
+            }
+            d.cpl = rows_cpl_;
+            d.rtab = rtab_;
+            d.rgeo = rgeo_;
+            d.tail = targs_;
+            d.tail.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+        }
+    }
+
     // ---- tail: the small coarse levels in one workgroup ----
     static constexpr int TAIL_NLMAX = (D == 3) ? (sizeof(S) == 8 ? 2 : 3) : (sizeof(S) == 8 ? 3 : 5);
     int tail_first_ = 0;        // first level of the tail (0: no tail)
@@ -500,7 +537,7 @@ public:
         Ctrl hc{};
         while (true) {
             int chunk = std::min(every, std::max(1, it_cap - queued));
-            for (int c = 0; c < chunk; c++) iteration(x, skip);
+            run_chunk(x, skip, chunk, every);
             queued += chunk;
             HIPCHK(hipMemcpyAsync(ctrl_host_, ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
             HIPCHK(hipStreamSynchronize(stream_));
@@ -540,6 +577,34 @@ public:
     unsigned flat_blocks(unsigned cap) const {
         long need = (N4_ / 4 + 255) / 256;
         return (unsigned)std::max<long>(1, std::min<long>(need, cap));
+    }
+
+    // `chunk` CG iterations; with use_graph the full-size chunk is captured
+    // once into a hipGraph and replayed (the graph holds kernel nodes only:
+    // the kernels read the device-resident scalars and the done flag)
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    int graph_chunk_ = 0;
+    void destroy_graph() {
+        if (gexec_) (void)hipGraphExecDestroy(gexec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        gexec_ = nullptr; graph_ = nullptr; graph_valid_ = false;
+    }
+    void run_chunk(T* x, const int* skip, int chunk, int every) {
+        if (o_.use_graph && !timer_.on && chunk == every) {
+            if (!graph_valid_ || graph_chunk_ != chunk) {
+                destroy_graph();
+                HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                for (int c = 0; c < chunk; c++) iteration(x, skip);
+                HIPCHK(hipStreamEndCapture(stream_, &graph_));
+                HIPCHK(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+                graph_valid_ = true;
+                graph_chunk_ = chunk;
+            }
+            HIPCHK(hipGraphLaunch(gexec_, stream_));
+            return;
+        }
+        for (int c = 0; c < chunk; c++) iteration(x, skip);
     }
 
     void iteration(T* x, const int* skip) {

@@ -29,9 +29,12 @@ namespace uaamg {
 
 template <typename T> using vec4 = T __attribute__((ext_vector_type(4)));
 
+// Grid-stride loop over 4-cell vectors.  fv_b / fv_n are the (virtual)
+// block index and block count: blockIdx.x / gridDim.x for a single grid, or a
+// work-list item's slice of one grid in the batched kernels.
 #define FLAT_LOOP(N4)                                                                          \
-    for (long v_ = (long)blockIdx.x * blockDim.x + threadIdx.x; v_ < (N4) / 4;                 \
-         v_ += (long)gridDim.x * blockDim.x)
+    for (long v_ = (long)fv_b * blockDim.x + threadIdx.x; v_ < (N4) / 4;                      \
+         v_ += (long)fv_n * blockDim.x)
 
 __device__ __forceinline__ uint8_t nbyte(uint32_t w, int q) { return (uint8_t)(w >> (8 * q)); }
 
@@ -63,8 +66,8 @@ __global__ void k_ctrl_init(Ctrl* ctrl, const double* count, int max_iter, doubl
 }
 
 // Sum of rhs over fluid cells -> m1 (the mean removed from the rhs).
-__global__ void k_init_sum(const double* __restrict__ rhs, const uint8_t* __restrict__ nbr, long N4,
-                           double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_init_sum(const double* __restrict__ rhs, const uint8_t* __restrict__ nbr, long N4,
+                           double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     double v[1] = {0.0};
     FLAT_LOOP(N4) {
         long c = 4 * v_;
@@ -76,16 +79,23 @@ __global__ void k_init_sum(const double* __restrict__ rhs, const uint8_t* __rest
     }
     block_store<1>(v, partials);
 }
-__global__ void k_fin_init_sum(const double* partials, int nb, Ctrl* ctrl) {
+__global__ void k_init_sum(const double* __restrict__ rhs, const uint8_t* __restrict__ nbr, long N4,
+                           double* partials, Ctrl* ctrl) {
+    d_init_sum(rhs, nbr, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void d_fin_init_sum(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     double tot[1];
-    reduce_partials<1>(partials, nb, tot);
+    reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) ctrl->m1 = (ctrl->project && ctrl->count > 0) ? tot[0] / ctrl->count : 0.0;
+}
+__global__ void k_fin_init_sum(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_init_sum(partials, nb, 0, nb, ctrl);
 }
 
 // r = -(rhs - m1) on fluid cells (-rhs without projection), 0 elsewhere; m2 = mean(r).
 template <typename T>
-__global__ void k_init_r(const double* __restrict__ rhs, T* __restrict__ r, const uint8_t* __restrict__ nbr,
-                         long N4, double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_init_r(const double* __restrict__ rhs, T* __restrict__ r, const uint8_t* __restrict__ nbr,
+                         long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     const double m1 = ctrl->m1;
     const int project = ctrl->project;
     double v[1] = {0.0};
@@ -104,15 +114,23 @@ __global__ void k_init_r(const double* __restrict__ rhs, T* __restrict__ r, cons
     }
     block_store<1>(v, partials);
 }
-__global__ void k_fin_init_r(const double* partials, int nb, Ctrl* ctrl) {
+template <typename T>
+__global__ void k_init_r(const double* __restrict__ rhs, T* __restrict__ r, const uint8_t* __restrict__ nbr,
+                         long N4, double* partials, Ctrl* ctrl) {
+    d_init_r<T>(rhs, r, nbr, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void d_fin_init_r(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     double tot[1];
-    reduce_partials<1>(partials, nb, tot);
+    reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) ctrl->m2 = (ctrl->project && ctrl->count > 0) ? tot[0] / ctrl->count : 0.0;
+}
+__global__ void k_fin_init_r(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_init_r(partials, nb, 0, nb, ctrl);
 }
 
 // r -= m2 on fluid cells; r0.r0 and the squared stopping threshold.
 template <typename T>
-__global__ void k_init_sub(T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4, double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_init_sub(T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     const double m2 = ctrl->m2;
     const int project = ctrl->project;
     double v[1] = {0.0};
@@ -132,14 +150,21 @@ __global__ void k_init_sub(T* __restrict__ r, const uint8_t* __restrict__ nbr, l
     }
     block_store<1>(v, partials);
 }
-__global__ void k_fin_init_sub(const double* partials, int nb, Ctrl* ctrl) {
+template <typename T>
+__global__ void k_init_sub(T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4, double* partials, Ctrl* ctrl) {
+    d_init_sub<T>(r, nbr, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void d_fin_init_sub(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     double tot[1];
-    reduce_partials<1>(partials, nb, tot);
+    reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) {
         ctrl->r0sq = tot[0];
         ctrl->rsnew = tot[0];
         ctrl->tolsq = (tot[0] > 0) ? tot[0] * ctrl->tol * ctrl->tol : ctrl->tol * ctrl->tol;
     }
+}
+__global__ void k_fin_init_sub(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_init_sub(partials, nb, 0, nb, ctrl);
 }
 
 // The PCG operator at one fluid cell, in FP64: uniform constants, ghost = self
@@ -190,8 +215,8 @@ __device__ __forceinline__ void apply_A4(const T* __restrict__ p, long c0, uint3
 
 // Ap = A p; pAp over fluid; alpha = rsold / pAp.  Non-fluid cells get Ap = 0.
 template <int D, typename T>
-__global__ void k_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const uint8_t* __restrict__ nbr, Geo g,
-                             Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const uint8_t* __restrict__ nbr, Geo g,
+                             Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     if (ctrl->done) return;
     double v[1] = {0.0};
     FLAT_LOOP(N4) {
@@ -214,10 +239,15 @@ __global__ void k_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const 
     }
     block_store<1>(v, partials);
 }
-__global__ void k_fin_matvec(const double* partials, int nb, Ctrl* ctrl) {
+template <int D, typename T>
+__global__ void k_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const uint8_t* __restrict__ nbr, Geo g,
+                             Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
+    d_matvec_dot<D, T>(p, Ap, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void d_fin_matvec(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     if (ctrl->done) return;
     double tot[1];
-    reduce_partials<1>(partials, nb, tot);
+    reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) {
         double pAp = tot[0];
         ctrl->pAp = pAp;
@@ -226,11 +256,14 @@ __global__ void k_fin_matvec(const double* partials, int nb, Ctrl* ctrl) {
         else ctrl->alpha = ctrl->rsold / pAp;
     }
 }
+__global__ void k_fin_matvec(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_matvec(partials, nb, 0, nb, ctrl);
+}
 
 // Warm start: r -= A x0 (after the rhs projection); r.r recomputed.
 template <int D, typename T>
-__global__ void k_init_warm(T* __restrict__ r, const T* __restrict__ x0, const uint8_t* __restrict__ nbr, Geo g,
-                            Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_init_warm(T* __restrict__ r, const T* __restrict__ x0, const uint8_t* __restrict__ nbr, Geo g,
+                            Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     double v[1] = {0.0};
     FLAT_LOOP(N4) {
         long c0 = 4 * v_;
@@ -251,9 +284,14 @@ __global__ void k_init_warm(T* __restrict__ r, const T* __restrict__ x0, const u
     }
     block_store<1>(v, partials);
 }
-__global__ void k_fin_init_warm(const double* partials, int nb, Ctrl* ctrl) {
+template <int D, typename T>
+__global__ void k_init_warm(T* __restrict__ r, const T* __restrict__ x0, const uint8_t* __restrict__ nbr, Geo g,
+                            Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
+    d_init_warm<D, T>(r, x0, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void d_fin_init_warm(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     double tot[1];
-    reduce_partials<1>(partials, nb, tot);
+    reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) {
         ctrl->rsnew = tot[0];
         // the stopping threshold stays relative to the cold-start residual
@@ -261,13 +299,16 @@ __global__ void k_fin_init_warm(const double* partials, int nb, Ctrl* ctrl) {
         if (ctrl->tol_mode == 1 && tot[0] < ctrl->tolsq) { ctrl->done = 1; ctrl->status = ST_CONVERGED; }
     }
 }
+__global__ void k_fin_init_warm(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_init_warm(partials, nb, 0, nb, ctrl);
+}
 
 // x += alpha p; r -= alpha Ap (fluid cells); r.r; convergence / iteration cap.
 // FLEX: also (r_new, z_old - mean_old) for the Polak-Ribiere beta.
 template <typename T, bool FLEX>
-__global__ void k_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
+__device__ __forceinline__ void d_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
                          const T* __restrict__ z, const uint8_t* __restrict__ nbr, long N4, double* partials,
-                         Ctrl* ctrl) {
+                         Ctrl* ctrl, int fv_b, int fv_n) {
     if (ctrl->done) return;
     const double alpha = ctrl->alpha;
     double v[3] = {0.0, 0.0, 0.0};
@@ -294,12 +335,18 @@ __global__ void k_update(T* __restrict__ x, T* __restrict__ r, const T* __restri
     if (FLEX) block_store<3>(v, partials);
     else { double v1[1] = {v[0]}; block_store<1>(v1, partials); }
 }
+template <typename T, bool FLEX>
+__global__ void k_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
+                         const T* __restrict__ z, const uint8_t* __restrict__ nbr, long N4, double* partials,
+                         Ctrl* ctrl) {
+    d_update<T, FLEX>(x, r, p, Ap, z, nbr, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
 template <bool FLEX>
-__global__ void k_fin_update(const double* partials, int nb, Ctrl* ctrl) {
+__device__ __forceinline__ void d_fin_update(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     if (ctrl->done) return;
     double tot[3] = {0.0, 0.0, 0.0};
-    if (FLEX) reduce_partials<3>(partials, nb, tot);
-    else { double t1[1]; reduce_partials<1>(partials, nb, t1); tot[0] = t1[0]; }
+    if (FLEX) reduce_partials_at<3>(partials, nbt, base, nb, tot);
+    else { double t1[1]; reduce_partials_at<1>(partials, nbt, base, nb, t1); tot[0] = t1[0]; }
     if (threadIdx.x == 0) {
         double rs = tot[0];
         ctrl->rsnew = rs;
@@ -311,6 +358,10 @@ __global__ void k_fin_update(const double* partials, int nb, Ctrl* ctrl) {
         else if (!isfinite(rs)) { ctrl->done = 1; ctrl->status = ST_NONFINITE; }
         else if (ctrl->iter >= ctrl->max_iter) { ctrl->done = 1; ctrl->status = ST_MAXITER; }
     }
+}
+template <bool FLEX>
+__global__ void k_fin_update(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_update<FLEX>(partials, nb, 0, nb, ctrl);
 }
 
 // Shared tail of the z reductions: mean of z, (r, z - mean), beta, rsold.
@@ -334,8 +385,8 @@ __device__ __forceinline__ void finish_zsums(Ctrl* ctrl, double sz, double srz, 
 
 // sum z, sum r.z, sum r over fluid cells (per-pass V-cycle path).
 template <typename T>
-__global__ void k_zsums(const T* __restrict__ z, const T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4,
-                        double* partials, Ctrl* ctrl) {
+__device__ __forceinline__ void d_zsums(const T* __restrict__ z, const T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4,
+                        double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     if (ctrl->done) return;
     double v[3] = {0.0, 0.0, 0.0};
     FLAT_LOOP(N4) {
@@ -352,18 +403,26 @@ __global__ void k_zsums(const T* __restrict__ z, const T* __restrict__ r, const 
     }
     block_store<3>(v, partials);
 }
+template <typename T>
+__global__ void k_zsums(const T* __restrict__ z, const T* __restrict__ r, const uint8_t* __restrict__ nbr, long N4,
+                        double* partials, Ctrl* ctrl) {
+    d_zsums<T>(z, r, nbr, N4, partials, ctrl, blockIdx.x, gridDim.x);
+}
 // finish of the z sums (k_zsums or the fused level-0 up kernel)
-__global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
+__device__ __forceinline__ void d_fin_zsums(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     if (ctrl->done) return;
     double tot[3];
-    reduce_partials<3>(partials, nb, tot);
+    reduce_partials_at<3>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) finish_zsums(ctrl, tot[0], tot[1], tot[2]);
+}
+__global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
+    d_fin_zsums(partials, nb, 0, nb, ctrl);
 }
 
 // p = (z - mean) + beta p on fluid cells.
 template <typename T>
-__global__ void k_pupdate(T* __restrict__ p, const T* __restrict__ z, const uint8_t* __restrict__ nbr, long N4,
-                          Ctrl* ctrl) {
+__device__ __forceinline__ void d_pupdate(T* __restrict__ p, const T* __restrict__ z, const uint8_t* __restrict__ nbr, long N4,
+                          Ctrl* ctrl, int fv_b, int fv_n) {
     if (ctrl->done) return;
     const double mean = ctrl->mean_z, beta = ctrl->beta;
     FLAT_LOOP(N4) {
@@ -376,6 +435,11 @@ __global__ void k_pupdate(T* __restrict__ p, const T* __restrict__ z, const uint
             if (nbyte(w, q) & NB_FLUID) pv[q] = (T)(((double)zv[q] - mean) + beta * (double)pv[q]);
         *(vec4<T>*)(p + c) = pv;
     }
+}
+template <typename T>
+__global__ void k_pupdate(T* __restrict__ p, const T* __restrict__ z, const uint8_t* __restrict__ nbr, long N4,
+                          Ctrl* ctrl) {
+    d_pupdate<T>(p, z, nbr, N4, ctrl, blockIdx.x, gridDim.x);
 }
 
 // p_out += x on fluid cells (cold start) or p_out = x (warm start).

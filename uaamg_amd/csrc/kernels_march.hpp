@@ -151,9 +151,9 @@ __device__ __forceinline__ MSten<D, L0, CT> mst(const MCell<L0, CT>& c, const MC
 
 // ------------------------------------------------------------------ down
 template <int D, bool L0, int RX, int RY, int PPT, typename VB, typename C4, typename VC, typename CC4>
-__global__ __launch_bounds__(256) void k_down_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
+__device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
                                                 VC* __restrict__ bc, const CC4* __restrict__ crec, Geo cg, int KC,
-                                                const int* skip) {
+                                                const int* skip, int bx, int by, int bz) {
     using CT = typename CTOf<VB>::type;
     using Cell = MCell<L0, CT>;
     constexpr int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1, RN = RX * RY;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_down_m(const VB* __restrict__ b, FusedA
     __shared__ typename std::conditional<std::is_same<CT, double>::value, double, float>::type sres[2][RN];
     __shared__ uint8_t sfl[2][RN];
     const Geo g = A.g;
-    const int a0 = blockIdx.x * TX + 1, b0 = (D == 3) ? blockIdx.y * TY + 1 : 0, m0 = blockIdx.z * KC + 1;
+    const int a0 = bx * TX + 1, b0 = (D == 3) ? by * TY + 1 : 0, m0 = bz * KC + 1;
     const int nm = mcount<D>(g);
     const long ms = mstride<D>(g);
     const int last = min(m0 + KC - 1, nm);
@@ -287,12 +287,18 @@ __global__ __launch_bounds__(256) void k_down_m(const VB* __restrict__ b, FusedA
     }
     (void)crec;
 }
+template <int D, bool L0, int RX, int RY, int PPT, typename VB, typename C4, typename VC, typename CC4>
+__global__ __launch_bounds__(256) void k_down_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
+                                                VC* __restrict__ bc, const CC4* __restrict__ crec, Geo cg, int KC,
+                                                const int* skip) {
+    d_down_m<D, L0, RX, RY, PPT, VB, C4, VC, CC4>(b, A, bc, crec, cg, KC, skip, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 // ------------------------------------------------------------------ up
 template <int D, bool L0, int RX, int RY, int PPT, bool REDUCE, typename VB, typename C4, typename VX, typename VCX>
-__global__ __launch_bounds__(256) void k_up_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
+__device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
                                               const VCX* __restrict__ xc, Geo cg, VX* __restrict__ xout, int KC,
-                                              const int* skip, double* partials) {
+                                              const int* skip, double* partials, int bx, int by, int bz) {
     using CT = typename CTOf<VB>::type;
     using Cell = MCell<L0, CT>;
     constexpr int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1;
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(256) void k_up_m(const VB* __restrict__ b, FusedArg
     if (skip && *skip) return;
     __shared__ MarchLDS<D, L0, RX, RY, CT> S;
     const Geo g = A.g;
-    const int a0 = blockIdx.x * TX + 1, b0 = (D == 3) ? blockIdx.y * TY + 1 : 0, m0 = blockIdx.z * KC + 1;
+    const int a0 = bx * TX + 1, b0 = (D == 3) ? by * TY + 1 : 0, m0 = bz * KC + 1;
     const int nm = mcount<D>(g), cnm = mcount<D>(cg);
     const long ms = mstride<D>(g), cms = mstride<D>(cg);
     const int last = min(m0 + KC - 1, nm);
@@ -418,6 +424,12 @@ __global__ __launch_bounds__(256) void k_up_m(const VB* __restrict__ b, FusedArg
     }
     if constexpr (REDUCE) block_store<3>(red, partials);
     (void)TY;
+}
+template <int D, bool L0, int RX, int RY, int PPT, bool REDUCE, typename VB, typename C4, typename VX, typename VCX>
+__global__ __launch_bounds__(256) void k_up_m(const VB* __restrict__ b, FusedArgs<D, L0, typename CTOf<VB>::type, C4> A,
+                                              const VCX* __restrict__ xc, Geo cg, VX* __restrict__ xout, int KC,
+                                              const int* skip, double* partials) {
+    d_up_m<D, L0, RX, RY, PPT, REDUCE, VB, C4, VX, VCX>(b, A, xc, cg, xout, KC, skip, partials, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 }  // namespace uaamg

@@ -62,14 +62,11 @@ __device__ __forceinline__ float lane_from_right(float v) {  // lane l gets lane
     return (threadIdx.x & 63) == 63 ? 0.0f : u;
 }
 
-// tile of this workgroup: consecutive block ids go round-robin to the 8 XCDs,
-// so block b runs on XCD b % 8; XCD x gets tiles [x*per_xcd, (x+1)*per_xcd)
-__device__ __forceinline__ int row_tile(const RowGeo& rg, int& ty, int& kc) {
-    const int b = blockIdx.x;
-    const int t = (b & 7) * rg.per_xcd + (b >> 3);
-    ty = t % rg.nty;
-    kc = t / rg.nty;
-    return t;
+// tile of block b of a single-grid launch: consecutive block ids go
+// round-robin to the 8 XCDs, so block b runs on XCD b % 8; XCD x gets tiles
+// [x*per_xcd, (x+1)*per_xcd)
+__device__ __forceinline__ int row_tile_of_block(const RowGeo& rg, int b) {
+    return (b & 7) * rg.per_xcd + (b >> 3);
 }
 
 __device__ __forceinline__ float rows_rinv(uint8_t nb, const float* srinv, const float* sd, const RowTab& T,
@@ -161,14 +158,14 @@ struct RowThread {
 // r -> coarse b (restricted residual after the forward sweep).  A workgroup
 // writes the coarse cells whose 8 children lie in its tile.
 template <int CPL, int NW, int RW, typename VC>
-__global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
-                                                        Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
-                                                        const int* skip) {
+__device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
+                                             const RowTab& K, const RowGeo& rg, VC* __restrict__ bc, const Geo& cg,
+                                             const int* skip, int tile) {
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
     if (skip && *skip) return;
     __shared__ RowLDS<CPL, NW, RW> L;
-    int ty, kc;
-    if (row_tile(rg, ty, kc) >= rg.ntiles) return;   // the whole workgroup leaves together
+    if (tile >= rg.ntiles) return;   // the whole workgroup leaves together
+    const int ty = tile % rg.nty, kc = tile / rg.nty;
     RowThread<CPL, NW, RW> th;
     th.init(g, rg, ty, kc);
     const int lane = th.lane, w = th.w;
@@ -316,18 +313,25 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
     }
 }
 
+template <int CPL, int NW, int RW, typename VC>
+__global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
+                                                        Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
+                                                        const int* skip) {
+    d_rows_down0<CPL, NW, RW, VC>(r, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x));
+}
+
 // ------------------------------------------------------------------ up
 // r, coarse x -> z (prolongation + reverse sweep), with sum z, sum r z, sum r
 // over fluid cells reduced per workgroup into partials.
 template <int CPL, int NW, int RW, typename VCX>
-__global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
-                                                      Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
-                                                      float* __restrict__ z, const int* skip, double* partials) {
+__device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
+                                           const RowTab& K, const RowGeo& rg, const VCX* __restrict__ xc, const Geo& cg,
+                                           float* __restrict__ z, const int* skip, double* partials, int tile) {
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
     if (skip && *skip) return;
     __shared__ RowLDS<CPL, NW, RW> L;
-    int ty, kc;
-    const bool active = row_tile(rg, ty, kc) < rg.ntiles;   // idle workgroups still store zero partials
+    const bool active = tile < rg.ntiles;   // idle workgroups still store zero partials
+    const int ty = tile % rg.nty, kc = tile / rg.nty;
     RowThread<CPL, NW, RW> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
@@ -476,6 +480,13 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
         }
     }
     block_store<3>(red, partials);
+}
+
+template <int CPL, int NW, int RW, typename VCX>
+__global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
+                                                      Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
+                                                      float* __restrict__ z, const int* skip, double* partials) {
+    d_rows_up0<CPL, NW, RW, VCX>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
 }
 
 }  // namespace uaamg

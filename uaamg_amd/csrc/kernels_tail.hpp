@@ -156,8 +156,8 @@ __device__ __forceinline__ void tail_restrict(const CT* Rs, CT* BC, CT* XC, cons
 }
 
 template <int D, typename S, typename C4, int NLMAX>
-__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs<C4> A, const S* __restrict__ b_first,
-                                                        S* __restrict__ x_first, const int* skip) {
+__device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restrict__ b_first, S* __restrict__ x_first,
+                                       const int* skip) {
     static_assert(NLMAX >= 2 && NLMAX <= TAIL_MAXL, "tail depth");
     using CT = typename CTOf<S>::type;
     if (skip && *skip) return;
@@ -235,6 +235,12 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs<C4> A, const S* 
         const CT* X0 = lds + A.xo[0];
         for (int e = tid; e < A.g[0].N; e += TAIL_THREADS) x_first[e] = (S)X0[e];
     }
+}
+
+template <int D, typename S, typename C4, int NLMAX>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs<C4> A, const S* __restrict__ b_first,
+                                                        S* __restrict__ x_first, const int* skip) {
+    d_tail<D, S, C4, NLMAX>(A, b_first, x_first, skip);
 }
 
 }  // namespace uaamg

@@ -6,6 +6,7 @@
 #include <memory>
 #include <string>
 #include "../../include/uaamg.h"
+#include "batch.hpp"
 #include "engine.hpp"
 #include "amgpcg.hpp"
 
@@ -475,4 +476,167 @@ extern "C" int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, dou
         std::fprintf(stderr, "uaamg_amgpcg: %s (%s)\n", hipGetErrorString(e.e), e.where.c_str());
         return e.e == hipErrorOutOfMemory ? UAAMG_ERR_OOM : UAAMG_ERR_HIP;
     }
+}
+
+// ------------------------------------------------------------------ batch C-ABI
+
+struct uaamg_batch {
+    uaamg_opts o{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<uaamg::BatchBase> part[2];       // [0]: 2D members, [1]: 3D members
+    std::vector<std::pair<int, int>> members;        // (part, index in part) in order of addition
+    uint8_t* stage = nullptr; long stage_n = 0;
+};
+
+static int bfail(uaamg_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BATCH_TRY(b, ...)                                                                          \
+    try { __VA_ARGS__ }                                                                            \
+    catch (const HipError& e) {                                                                    \
+        return bfail(b, e.e == hipErrorOutOfMemory ? UAAMG_ERR_OOM : UAAMG_ERR_HIP,               \
+                     std::string(hipGetErrorString(e.e)) + " (" + e.where + ")");                 \
+    }                                                                                              \
+    catch (const std::bad_alloc&) { return bfail(b, UAAMG_ERR_OOM, "host allocation failed"); }
+
+static std::unique_ptr<uaamg::BatchBase> make_batch_part(int dim, const uaamg_opts& o, hipStream_t s) {
+    if (dim == 2) {
+        if (o.precision == UAAMG_FP32) return std::make_unique<uaamg::BatchEngine<2, float>>(o, s);
+        return std::make_unique<uaamg::BatchEngine<2, __bf16>>(o, s);
+    }
+    if (o.precision == UAAMG_FP32) return std::make_unique<uaamg::BatchEngine<3, float>>(o, s);
+    return std::make_unique<uaamg::BatchEngine<3, __bf16>>(o, s);
+}
+
+extern "C" int uaamg_batch_create(uaamg_batch** out, const uaamg_opts* o) {
+    if (!out) return UAAMG_ERR_ARG;
+    *out = nullptr;
+    uaamg_opts d;
+    uaamg_default_opts(&d);
+    d.precision = UAAMG_FP32;
+    if (!o) o = &d;
+    std::string why;
+    if (!opts_valid(o, why)) return UAAMG_ERR_ARG;
+    if (o->precision != UAAMG_FP32 && o->precision != UAAMG_MIXED_BF16) return UAAMG_ERR_ARG;
+    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused) return UAAMG_ERR_ARG;
+    int rc = check_device(why);
+    if (rc) { std::fprintf(stderr, "uaamg_batch_create: %s\n", why.c_str()); return rc; }
+    uaamg_batch* b = new (std::nothrow) uaamg_batch();
+    if (!b) return UAAMG_ERR_OOM;
+    b->o = *o;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return UAAMG_ERR_HIP; }
+    *out = b;
+    return UAAMG_OK;
+}
+
+extern "C" void uaamg_batch_destroy(uaamg_batch* b) {
+    if (!b) return;
+    b->part[0].reset();
+    b->part[1].reset();
+    if (b->stage) (void)hipFree(b->stage);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" const char* uaamg_batch_last_error(const uaamg_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+static int batch_add_impl(uaamg_batch* b, const uaamg_grid* g, int* id, bool device_mask) {
+    if (!b) return UAAMG_ERR_ARG;
+    std::string why;
+    if (!grid_valid(g, why)) return bfail(b, UAAMG_ERR_ARG, why);
+    BATCH_TRY(b, {
+        const int pi = g->dim == 3 ? 1 : 0;
+        if (!b->part[pi]) b->part[pi] = make_batch_part(g->dim, b->o, b->stream);
+        const long N = (long)(g->n[0] + 2) * (g->n[1] + 2) * (g->dim == 3 ? (g->n[2] + 2) : 1);
+        const uint8_t* src = g->solid;
+        if (!device_mask) {
+            if (b->stage_n < N) {
+                if (b->stage) HIPCHK(hipFree(b->stage));
+                b->stage = nullptr;
+                HIPCHK(hipMalloc(&b->stage, N));
+                b->stage_n = N;
+            }
+            HIPCHK(hipMemcpyAsync(b->stage, g->solid, N, hipMemcpyHostToDevice, b->stream));
+            src = b->stage;
+        }
+        int n[3] = {g->n[0], g->n[1], g->dim == 3 ? g->n[2] : 1};
+        double h[3] = {g->h[0], g->h[1], g->dim == 3 ? g->h[2] : 1.0};
+        int wall[6] = {0, 0, 0, 0, 0, 0};
+        for (int f = 0; f < 2 * g->dim; f++) wall[f] = g->wall[f];
+        const int local = b->part[pi]->add(n, h, wall, src);
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->members.emplace_back(pi, local);
+        if (id) *id = (int)b->members.size() - 1;
+    })
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_batch_add(uaamg_batch* b, const uaamg_grid* g, int* id) { return batch_add_impl(b, g, id, false); }
+extern "C" int uaamg_batch_add_device(uaamg_batch* b, const uaamg_grid* g, int* id) { return batch_add_impl(b, g, id, true); }
+
+extern "C" int uaamg_batch_clear(uaamg_batch* b) {
+    if (!b) return UAAMG_ERR_ARG;
+    BATCH_TRY(b, {
+        for (auto& p : b->part) if (p) p->clear();
+        b->members.clear();
+    })
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_batch_size(const uaamg_batch* b) { return b ? (int)b->members.size() : 0; }
+
+extern "C" size_t uaamg_batch_bytes(const uaamg_batch* b) {
+    if (!b) return 0;
+    size_t s = 0;
+    for (auto& p : b->part) if (p) s += p->bytes();
+    return s;
+}
+
+extern "C" int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, double* const* p_dev,
+                                 const int* warm_start, int max_iter, double tol, uaamg_stats* st) {
+    if (!b || !rhs_dev || !p_dev) return UAAMG_ERR_ARG;
+    const int n = (int)b->members.size();
+    for (int i = 0; i < n; i++)
+        if (!rhs_dev[i] || !p_dev[i]) return bfail(b, UAAMG_ERR_ARG, "null rhs or p pointer");
+    if (max_iter < 0) max_iter = b->o.max_iter;
+    if (tol < 0) tol = b->o.tol;
+    BATCH_TRY(b, {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<const double*> rhs[2];
+        std::vector<double*> pout[2];
+        std::vector<int> warm[2];
+        for (int i = 0; i < n; i++) {
+            const int pi = b->members[i].first;
+            rhs[pi].push_back(rhs_dev[i]);
+            pout[pi].push_back(p_dev[i]);
+            warm[pi].push_back(warm_start ? warm_start[i] : 0);
+        }
+        for (int pi = 0; pi < 2; pi++)
+            if (b->part[pi] && b->part[pi]->size())
+                b->part[pi]->begin(rhs[pi].data(), pout[pi].data(), warm[pi].data(), max_iter, tol);
+        auto busy = [&]() {
+            for (auto& p : b->part) if (p && p->size() && !p->done()) return true;
+            return false;
+        };
+        while (busy()) {
+            for (auto& p : b->part) if (p && p->size()) p->chunk();
+            HIPCHK(hipStreamSynchronize(b->stream));
+            for (auto& p : b->part) if (p && p->size()) p->poll();
+        }
+        for (auto& p : b->part) if (p && p->size()) p->finish();
+        HIPCHK(hipStreamSynchronize(b->stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        int worst = UAAMG_OK;
+        for (int i = 0; i < n; i++) {
+            uaamg_stats tmp;
+            b->part[b->members[i].first]->stats(b->members[i].second, &tmp, ms);
+            if (st) st[i] = tmp;
+            if (tmp.status != UAAMG_OK && worst == UAAMG_OK) worst = tmp.status;
+        }
+        return worst;
+    })
+    return UAAMG_OK;
 }

@@ -192,7 +192,7 @@ def _random_mask(rng, n, frac_max=0.3):
     dim = len(n)
     shape = tuple(v + 2 for v in n)[::-1]
     axes = [(np.arange(s) - 0.5) / (s - 2) for s in shape]
-    grids = np.meshgrid(*axes, indexing="ij")            # grids[0] is the slowest axis
+    grids = np.meshgrid(*axes, indexing="ij", sparse=True)   # broadcastable; grids[0] is the slowest axis
     m = np.zeros(shape, bool)
     target = rng.uniform(0.0, frac_max)
     for _ in range(64):
@@ -218,13 +218,15 @@ def _vortex_rhs(rng, n, solid, nvort=16):
     shape = tuple(v + 2 for v in n)[::-1]
     s = solid.reshape(shape).astype(bool)
     axes = [(np.arange(sh) - 0.5) / (sh - 2) for sh in shape]
-    grids = np.meshgrid(*axes, indexing="ij")
+    grids = np.meshgrid(*axes, indexing="ij", sparse=True)
     vel = [np.zeros(shape) for _ in range(dim)]
     for _ in range(nvort):
         cc = rng.uniform(0, 1, dim)
         sg = rng.uniform(0.03, 0.15)
         a = rng.uniform(-1, 1)
-        e = np.exp(-sum((grids[q] - cc[q]) ** 2 for q in range(dim)) / sg ** 2)
+        e = 1.0
+        for q in range(dim):            # separable Gaussian: a product of 1D factors
+            e = e * np.exp(-(grids[q] - cc[q]) ** 2 / sg ** 2)
         # rotation in the plane of the two fastest axes, plus a source term so div != 0
         vel[-1] += a * (grids[-2] - cc[-2]) * e
         vel[-2] -= a * (grids[-1] - cc[-1]) * e
@@ -259,10 +261,37 @@ def batch_mixed(seed=0, n2d=64, n3d=32, side2d=(64, 1024), side3d=(32, 160)):
         wall = [int(rng.random() < 0.25) for _ in range(6)]
         rhs = _vortex_rhs(rng, n, solid)
         pb = Problem(n, (1.0,) * dim, solid, rhs, wall=wall, name=f"batch_{dim}d_{side}")
-        fl = pb.fluid_mask()
-        if all(w == NEUMANN for w in wall[:2 * dim]):
-            _zero_mean(pb.rhs, fl)
-        else:
-            pb.rhs[~fl] = 0.0
+        _consistent_rhs(pb)
         out.append(pb)
     return out
+
+
+def _consistent_rhs(pb):
+    """Make the rhs solvable: zero on non-fluid cells and zero-mean on every
+    connected fluid region that touches no Dirichlet wall (a random mask can
+    cut off pockets of fluid; each such pocket carries its own constant)."""
+    from scipy import ndimage
+    fl = pb.fluid_mask()
+    F = pb.view(fl)
+    lab, nlab = ndimage.label(F)
+    rhs = pb.rhs
+    rhs[~fl] = 0.0
+    if nlab == 0:
+        return
+    L = lab.reshape(-1)
+    touches = np.zeros(nlab + 1, bool)
+    V = lab
+    for f in range(2 * pb.dim):
+        if pb.wall[f] != DIRICHLET:
+            continue
+        ax = pb.dim - 1 - f // 2                     # face -x,+x,-y,+y,-z,+z -> array axis ([k][j][i])
+        idx = [slice(None)] * pb.dim
+        idx[ax] = 1 if f % 2 == 0 else V.shape[ax] - 2
+        touches[np.unique(V[tuple(idx)])] = True
+    cnt = np.bincount(L, minlength=nlab + 1).astype(float)
+    sm = np.bincount(L, weights=rhs, minlength=nlab + 1)
+    mean = np.where(cnt > 0, sm / np.maximum(cnt, 1), 0.0)
+    mean[0] = 0.0
+    mean[touches] = 0.0
+    rhs -= mean[L]
+    rhs[~fl] = 0.0
