@@ -211,6 +211,196 @@ def bench_batch(args):
     return res
 
 
+def _serve_sims(nsim, seed, dev):
+    """Simulations of the served stream: half 2D (side log-uniform 64..2048),
+    half 3D (side log-uniform 32..256), all walls Neumann, up to 3 disjoint
+    solid balls away from the walls (the fluid stays connected), generated on
+    the device."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(nsim):
+        dim = 2 if k % 2 == 0 else 3
+        lo, hi = (64, 2048) if dim == 2 else (32, 256)
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        n = (side,) * dim
+        shape = tuple(v + 2 for v in n)[::-1]
+        ax = [((torch.arange(s, device=dev, dtype=torch.float32) - 0.5) / (s - 2)) for s in shape]
+        X = torch.meshgrid(*ax, indexing="ij")
+        solid = torch.zeros(shape, dtype=torch.bool, device=dev)
+        balls = []
+        for _ in range(int(rng.integers(0, 4))):
+            for _try in range(20):
+                r = rng.uniform(0.04, 0.1)
+                c = rng.uniform(r + 0.05, 1 - r - 0.05, dim)
+                if all(np.linalg.norm(c - c2) > r + r2 + 0.05 for c2, r2 in balls):
+                    balls.append((c, r))
+                    break
+        for c, r in balls:
+            d2 = sum((X[q] - float(c[dim - 1 - q])) ** 2 for q in range(dim))
+            solid |= d2 < r * r
+        inner = (slice(1, -1),) * dim
+        fluid = torch.zeros(shape, dtype=torch.bool, device=dev)
+        fluid[inner] = True
+        fluid &= ~solid
+        # smooth rhs: a few Gaussian sources and sinks, zero mean over the fluid
+        rhs = torch.zeros(shape, dtype=torch.float64, device=dev)
+        for _ in range(6):
+            c = rng.uniform(0.1, 0.9, dim)
+            s = rng.uniform(0.05, 0.2)
+            rhs += rng.uniform(-1e3, 1e3) * torch.exp(-sum((X[q].double() - float(c[dim - 1 - q])) ** 2 for q in range(dim)) / s ** 2)
+        rhs[~fluid] = 0
+        rhs[fluid] -= rhs[fluid].mean()
+        sims.append({"n": n, "h": tuple(1.0 / v for v in n), "solid": solid.reshape(-1).to(torch.uint8).cpu().numpy(),
+                     "fluid": fluid.reshape(-1), "rhs": rhs.reshape(-1), "shape": shape, "dim": dim})
+    return sims
+
+
+def _next_frame(sim, rng):
+    """Frame t+1: frame t's rhs advected by a smooth (one-cell) shift plus 5 % new noise."""
+    import torch
+    r = sim["rhs"].reshape(sim["shape"])
+    ax = int(rng.integers(0, sim["dim"]))
+    r = torch.roll(r, shifts=int(rng.choice([-1, 1])), dims=ax)
+    noise = torch.randn(r.shape, dtype=torch.float64, device=r.device)
+    noise = noise * (0.05 * float(r.abs().max()))
+    r = r + noise
+    r = r.reshape(-1)
+    f = sim["fluid"]
+    r[~f] = 0
+    r[f] -= r[f].mean()
+    sim["rhs"] = r
+    return r
+
+
+def bench_serve(args):
+    """Served stream: 200 concurrent simulations submit one pressure solve per
+    frame (relres 1e-5, warm start from their previous pressure).  Closed
+    loop first (every simulation always has a frame queued) to measure the
+    capacity, then Poisson arrivals at 75 % of it; latency = submit ->
+    completion.  Errors of sampled requests against an FP64 GPU solve of the
+    same rhs (the FP64 path matches the FP64 oracle to 1e-9)."""
+    import torch
+    import uaamg_amd as U
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    nsim = args.serve_sims
+    sims = _serve_sims(nsim, 7, dev)
+    cells = sum(int(np.prod(s["n"])) for s in sims)
+    log(f"[serve] generated {nsim} simulations ({cells / 1e6:.1f} M cells) in {time.perf_counter() - t0:.1f}s")
+    budget = int(args.serve_budget_gb * (1 << 30))
+    srv = U.Server(budget, max_active=args.serve_slots, precision=U.MIXED_BF16, tol_mode=U.TOL_REL, use_graph=1,
+                   max_iter=500)
+    for k, s in enumerate(sims):
+        srv.register(k, s["n"], s["h"], s["solid"])
+    p = [torch.zeros(int(np.prod(s["shape"])), dtype=torch.float64, device=dev) for s in sims]
+    rng = np.random.default_rng(11)
+    tol = 1e-5
+    frames = [0] * nsim
+
+    def submit(k, keep=None):
+        r = _next_frame(sims[k], rng) if frames[k] else sims[k]["rhs"]
+        r = r.clone()
+        if keep is not None:
+            keep.append(r)
+        req = srv.submit(k, r.data_ptr(), p[k].data_ptr(), 1 if frames[k] else 0, 500, tol)
+        frames[k] += 1
+        return req, r
+
+    # closed loop: warm-up pass over every simulation, then timed
+    live = {}
+    for k in range(nsim):
+        req, r = submit(k)
+        live[req] = (k, r)
+    torch.cuda.synchronize()
+    t_closed = time.perf_counter()
+    done_closed = 0
+    while time.perf_counter() - t_closed < args.serve_seconds:
+        for d in srv.step():
+            k, _ = live.pop(d)
+            done_closed += 1
+            req, r = submit(k)
+            live[req] = (k, r)
+    cap = done_closed / (time.perf_counter() - t_closed)
+    while live:
+        for d in srv.step():
+            live.pop(d, None)
+    log(f"[serve] closed-loop capacity {cap:.1f} solves/s ({done_closed} solves)")
+
+    # open loop: Poisson arrivals at 75 % of capacity
+    lam = 0.75 * cap
+    nreq = max(50, int(lam * args.serve_seconds))
+    arr = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    order = rng.integers(0, nsim, nreq)
+    pending_sim = {k: [] for k in range(nsim)}
+    t_start = time.perf_counter()
+    nxt = 0
+    live = {}
+    lat = []
+    iters = []
+    sample = {}
+    while nxt < nreq or live:
+        now = time.perf_counter() - t_start
+        while nxt < nreq and arr[nxt] <= now:
+            k = int(order[nxt])
+            keep = [] if len(sample) < 8 and nxt % 7 == 0 else None
+            req, r = submit(k, keep)
+            live[req] = (k, time.perf_counter())
+            if keep is not None:
+                sample[req] = (k, r, frames[k])
+            nxt += 1
+        for d in srv.step():
+            k, ts = live.pop(d)
+            st = srv.request(d)
+            lat.append(st.ms_total)
+            iters.append(st.iters)
+            if d in sample:
+                sample[d] = sample[d] + (p[k].clone(),)
+        if not live and nxt < nreq:
+            time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t_start))))
+    wall = time.perf_counter() - t_start
+    torch.cuda.synchronize()
+    # errors of sampled requests vs an FP64 solve of the same rhs
+    errs = []
+    for req, item in sample.items():
+        if len(item) < 4:
+            continue
+        k, r, fr, pserved = item
+        s = sims[k]
+        c = U.Context(precision=U.FP64, tol_mode=U.TOL_REL)
+        c.set_grid(s["n"], s["h"], s["solid"])
+        pref = torch.zeros_like(r)
+        c.solve_device(r.data_ptr(), pref.data_ptr(), 0, 1000, 1e-10)
+        c.close()
+        f = s["fluid"]
+        a, b = pserved[f], pref[f]
+        a = a - a.mean()
+        b = b - b.mean()
+        errs.append(float((a - b).abs().max() / b.abs().max()))
+    st = srv.status()
+    srv.close()
+    lat = np.array(lat)
+    res = {
+        "workload": f"serving: {nsim} concurrent simulations (2D side logU[64,2048], 3D side logU[32,256], "
+                    "Neumann walls, 0-3 solid balls), frame rhs = previous rhs shifted one cell + 5 % noise, "
+                    "relres 1e-5, warm start, BF16 levels; Poisson arrivals at 75 % of the closed-loop capacity",
+        "capacity_solves_per_s": round(cap, 2),
+        "arrival_rate_per_s": round(lam, 2),
+        "requests": int(nreq),
+        "solves_per_s": round(nreq / wall, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_iters": round(float(np.mean(iters)), 2),
+        "hbm_budget_gb": args.serve_budget_gb,
+        "slots": args.serve_slots,
+        "max_rel_err_vs_fp64_sampled": max(errs) if errs else None,
+        "sampled": len(errs),
+        "resident_at_end": st["resident"],
+    }
+    log(f"[serve] {json.dumps(res)}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +412,12 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-size stream")
     ap.add_argument("--batch-only", action="store_true", help="only the batched mixed-size stream")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--serve-only", action="store_true", help="only the served stream")
+    ap.add_argument("--serve-sims", type=int, default=200)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0)
+    ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (3x), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -234,6 +430,9 @@ def main():
 
     if args.batch_only:
         print(json.dumps({"batched": bench_batch(args)}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": bench_serve(args)}), flush=True)
         return
 
     import torch
@@ -358,6 +557,8 @@ def main():
         del rhs_d, solid_d, p_d
         ctx.close()
         out["batched"] = bench_batch(args)
+    if not args.no_serve:
+        out["serving"] = bench_serve(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(pb, k_med)

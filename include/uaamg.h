@@ -182,6 +182,29 @@ size_t uaamg_batch_bytes(const uaamg_batch* b);
 int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, double* const* p_dev,
                       const int* warm_start, int max_iter, double tol, uaamg_stats* st);
 
+/* ---- serving: pressure solves of many concurrent simulations ----
+ * Each request is one Solver::solve of one simulation frame
+ * (include/solver/solver.h:29-30); the loop keeps each simulation's hierarchy
+ * resident while the HBM budget allows (least recently used idle simulations
+ * are evicted), runs every active request in one batch and admits new
+ * requests between chunks of iterations (continuous batching).  With
+ * warm_start = 1 the request's p_dev holds the previous frame's pressure and
+ * receives the new one.  precision: UAAMG_FP32 or UAAMG_MIXED_BF16. */
+typedef struct uaamg_server uaamg_server;
+int uaamg_server_create(uaamg_server** out, const uaamg_opts* o, size_t hbm_budget, int max_active);
+void uaamg_server_destroy(uaamg_server* s);
+const char* uaamg_server_last_error(const uaamg_server* s);
+/* Register simulation `sim` with its grid (host mask, copied). */
+int uaamg_server_register(uaamg_server* s, int sim, const uaamg_grid* g);
+/* Queue one solve of `sim` (device arrays); *req = request id. */
+int uaamg_server_submit(uaamg_server* s, int sim, const double* rhs_dev, double* p_dev, int warm_start,
+                        int max_iter, double tol, int* req);
+/* Admit, run one chunk, complete: done[0..*n_done) = finished request ids. */
+int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* n_done);
+/* Stats of a finished request (ms_total = latency from submit to completion). */
+int uaamg_server_request(const uaamg_server* s, int req, uaamg_stats* st);
+int uaamg_server_status(const uaamg_server* s, int* queued, int* active, int* resident, size_t* bytes_used);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,7 +45,9 @@ EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_er
            "uaamg_amgpcg", "uaamg_bytes_required",
            "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
            "uaamg_batch_add_device", "uaamg_batch_clear", "uaamg_batch_size", "uaamg_batch_bytes",
-           "uaamg_batch_solve"]
+           "uaamg_batch_solve",
+           "uaamg_server_create", "uaamg_server_destroy", "uaamg_server_last_error", "uaamg_server_register",
+           "uaamg_server_submit", "uaamg_server_step", "uaamg_server_request", "uaamg_server_status"]
 
 
 def lib_path():
@@ -90,6 +92,16 @@ class Lib:
         L.uaamg_batch_bytes.argtypes = [P]
         L.uaamg_batch_bytes.restype = C.c_size_t
         L.uaamg_batch_solve.argtypes = [P, P, P, P, I, D, P]
+        L.uaamg_server_create.argtypes = [C.POINTER(C.c_void_p), P, C.c_size_t, I]
+        L.uaamg_server_destroy.argtypes = [P]
+        L.uaamg_server_destroy.restype = None
+        L.uaamg_server_last_error.argtypes = [P]
+        L.uaamg_server_last_error.restype = C.c_char_p
+        L.uaamg_server_register.argtypes = [P, I, P]
+        L.uaamg_server_submit.argtypes = [P, I, P, P, I, I, D, P]
+        L.uaamg_server_step.argtypes = [P, P, I, P]
+        L.uaamg_server_request.argtypes = [P, I, P]
+        L.uaamg_server_status.argtypes = [P, P, P, P, P]
 
     def default_opts(self, **kw):
         o = Opts()
@@ -304,3 +316,64 @@ class Batch:
         if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
             self._err("uaamg_batch_solve", rc)
         return rc, list(st)
+
+
+class Server:
+    """Request loop over continuous batches (uaamg_server_*)."""
+
+    def __init__(self, budget_bytes, max_active=64, lib=None, **opts):
+        self.lib = lib or load()
+        opts.setdefault("precision", FP32)
+        self.opts = self.lib.default_opts(**opts)
+        h = C.c_void_p()
+        rc = self.lib.L.uaamg_server_create(C.byref(h), C.byref(self.opts), int(budget_bytes), int(max_active))
+        if rc != OK:
+            raise RuntimeError(f"uaamg_server_create failed: {status_name(rc)}")
+        self.h = h
+        self._done = (C.c_int * 4096)()
+
+    h = None
+
+    def close(self):
+        if self.h:
+            self.lib.L.uaamg_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _err(self, what, rc):
+        msg = self.lib.L.uaamg_server_last_error(self.h).decode()
+        raise RuntimeError(f"{what}: {status_name(rc)}: {msg}")
+
+    def register(self, sim, n, h, solid, wall=None):
+        solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        g = make_grid(n, h, solid, wall)
+        rc = self.lib.L.uaamg_server_register(self.h, int(sim), C.byref(g))
+        if rc != OK:
+            self._err("uaamg_server_register", rc)
+
+    def submit(self, sim, rhs_ptr, p_ptr, warm=0, max_iter=-1, tol=-1.0):
+        r = C.c_int(-1)
+        rc = self.lib.L.uaamg_server_submit(self.h, int(sim), C.c_void_p(int(rhs_ptr)), C.c_void_p(int(p_ptr)),
+                                            int(warm), int(max_iter), float(tol), C.byref(r))
+        if rc != OK:
+            self._err("uaamg_server_submit", rc)
+        return r.value
+
+    def step(self):
+        n = C.c_int(0)
+        rc = self.lib.L.uaamg_server_step(self.h, self._done, len(self._done), C.byref(n))
+        if rc != OK:
+            self._err("uaamg_server_step", rc)
+        return [self._done[k] for k in range(n.value)]
+
+    def request(self, req):
+        st = Stats()
+        rc = self.lib.L.uaamg_server_request(self.h, int(req), C.byref(st))
+        return st if rc == OK else None
+
+    def status(self):
+        q, a, r = C.c_int(0), C.c_int(0), C.c_int(0)
+        b = C.c_size_t(0)
+        self.lib.L.uaamg_server_status(self.h, C.byref(q), C.byref(a), C.byref(r), C.byref(b))
+        return {"queued": q.value, "active": a.value, "resident": r.value, "bytes": b.value}

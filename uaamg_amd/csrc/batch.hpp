@@ -38,6 +38,13 @@ public:
     virtual void finish() = 0;
     virtual void stats(int g, uaamg_stats* st, double ms_total) const = 0;
     virtual int tuned_target() const = 0;
+    virtual void prepare() = 0;
+    virtual void start_member(int g, const double* rhs, double* pout, int warm, int max_iter, double tol) = 0;
+    virtual void finish_member(int g, double* pout, int warm) = 0;
+    virtual void run_chunk() = 0;
+    virtual const Ctrl& member_ctrl(int g) const = 0;
+    virtual void remove(int g) = 0;
+    virtual size_t member_bytes(int g) const = 0;
 };
 
 template <int D, typename S>
@@ -60,6 +67,7 @@ public:
     int add(const int n[3], const double h[3], const int wall[6], const uint8_t* solid_dev) override {
         auto e = std::make_unique<E>(o_, n, h, wall, s_);
         e->set_grid(solid_dev);
+        e->set_idle();
         if (!e->batch_ready())
             throw HipError{hipErrorInvalidValue, "grid cannot join a batch (needs V(1,1), >= 2 levels and a tail level)"};
         eng_.push_back(std::move(e));
@@ -137,6 +145,10 @@ public:
             for (int c = 0; c < n; c++) iteration();
         }
         queued_ += n;
+        copy_ctrl();
+    }
+    void copy_ctrl() {
+        kb_gather_ctrl<S><<<(size() + 63) / 64, 64, 0, s_>>>(dd_, size(), ctrl_);
         HIPCHK(hipMemcpyAsync(hctrl_, ctrl_, sizeof(Ctrl) * size(), hipMemcpyDeviceToHost, s_));
     }
     // after the stream has been synchronised
@@ -149,7 +161,7 @@ public:
     void finish() override {
         if (eng_.empty()) return;
         kb_finalize<S><<<nflat_, 256, 0, s_>>>(dd_, flat_);
-        HIPCHK(hipMemcpyAsync(hctrl_, ctrl_, sizeof(Ctrl) * size(), hipMemcpyDeviceToHost, s_));
+        copy_ctrl();
         HIPCHK(hipGetLastError());
     }
     // after finish() and a synchronisation
@@ -167,6 +179,42 @@ public:
         st->alg_bytes = eng_[g]->alg_bytes_of(c.iter);
     }
     int tuned_target() const override { return target_; }
+
+    // ---- continuous batching: members start and finish on their own ----
+    // (used by the server: members join between chunks, converged members
+    // leave; idle members skip every batched launch)
+    void prepare() override {
+        if (dirty_) build();
+    }
+    void start_member(int g, const double* rhs, double* pout, int warm, int max_iter, double tol) override {
+        eng_[g]->start(rhs, pout, warm, max_iter, tol);
+    }
+    void finish_member(int g, double* pout, int warm) override { eng_[g]->finalize_into(pout, warm); }
+    void run_chunk() override {
+        if (eng_.empty()) return;
+        if (dirty_) build();
+        const int every = std::max(1, o_.check_every);
+        if (o_.use_graph) {
+            if (!gexec_) {
+                HIPCHK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+                for (int c = 0; c < every; c++) iteration();
+                HIPCHK(hipStreamEndCapture(s_, &graph_));
+                HIPCHK(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(gexec_, s_));
+        } else {
+            for (int c = 0; c < every; c++) iteration();
+        }
+        copy_ctrl();
+    }
+    const Ctrl& member_ctrl(int g) const override { return hctrl_[g]; }
+    void remove(int g) override {
+        HIPCHK(hipStreamSynchronize(s_));
+        destroy_graph();
+        eng_.erase(eng_.begin() + g);
+        dirty_ = true;
+    }
+    size_t member_bytes(int g) const override { return eng_[g]->bytes(); }
 
 private:
     // ---- one PCG iteration of every member ----
@@ -216,7 +264,7 @@ private:
         dd_ = dev<BGrid<S>>(G);
         keep_ = lists_.size();   // ctrl and descriptors stay; work-lists are rebuilt by the autotuner
         hd_.assign(G, BGrid<S>{});
-        for (int g = 0; g < G; g++) eng_[g]->fill_batch(hd_[g], ctrl_ + g);
+        for (int g = 0; g < G; g++) eng_[g]->fill_batch(hd_[g], eng_[g]->ctrl_ptr());
         make_lists(target_);
         dirty_ = false;
         if (!tuned_) autotune();
@@ -326,6 +374,8 @@ private:
         free_lists();
         hd_ = save;
         make_lists(target_);
+        kb_ctrl_init<S><<<(G + 63) / 64, 64, 0, s_>>>(dd_, G, 0, 0.0, o_.tol_mode, 0, o_.project_mean);   // idle again
+        HIPCHK(hipStreamSynchronize(s_));
     }
 
     void destroy_graph() {

@@ -319,26 +319,33 @@ public:
     static constexpr int TAIL_NLMAX = (D == 3) ? (sizeof(S) == 8 ? 2 : 3) : (sizeof(S) == 8 ? 3 : 5);
     int tail_first_ = 0;        // first level of the tail (0: no tail)
     TailArgs<C4> targs_{};
-    void tail_setup() {
-        tail_first_ = 0;
-        if (const char* env = std::getenv("UAAMG_TAIL")) if (env[0] == '0') return;
-        const int nl = nlevels();
+    // First level of the tail kernel for a hierarchy (0: none fits): the
+    // largest tail whose first level has <= 2048 cell pairs, deeper levels
+    // <= 1024, at most TAIL_NLMAX levels and x, b, residual within the LDS.
+    static int choose_tail(const std::vector<Geo>& shape) {
+        const int nl = (int)shape.size();
         auto pairs = [&](int l) {
-            const Geo& g = shape_[l];
+            const Geo& g = shape[l];
             return (long)((g.nx + 1) / 2) * g.ny * (D == 3 ? g.nz : 1);
         };
-        for (int lt = 1; lt <= nl - 2; lt++) {
+        for (int lt = 1; lt <= nl - 1; lt++) {
             if (nl - lt > TAIL_NLMAX) continue;
             if (pairs(lt) > (long)TAIL_MAXP * TAIL_THREADS) continue;
             bool fit = true;
             for (int l = lt + 1; l < nl; l++) fit = fit && pairs(l) <= TAIL_THREADS;
             long elems = 0;
-            for (int l = lt; l < nl; l++) elems += 2 * shape_[l].N;
-            elems += shape_[lt].N;
+            for (int l = lt; l < nl; l++) elems += 2 * shape[l].N;
+            elems += shape[lt].N;
             if (!fit || elems * (long)sizeof(SC) > TAIL_LDS_BYTES) continue;
-            tail_first_ = lt;
-            break;
+            return lt;
         }
+        return 0;
+    }
+    void tail_setup() {
+        tail_first_ = 0;
+        if (const char* env = std::getenv("UAAMG_TAIL")) if (env[0] == '0') return;
+        const int nl = nlevels();
+        tail_first_ = choose_tail(shape_);
         if (!tail_first_) return;
         targs_.nl = nl - tail_first_;
         int off = 0;
@@ -491,11 +498,9 @@ public:
     }
 
     // ---------------- solve ----------------
-    int solve(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, uaamg_stats* st) override {
-        auto t0 = std::chrono::steady_clock::now();
-        timer_.on = (o_.timing != 0);
-        timer_.mask = (unsigned)o_.timing;
-        timer_.reset();
+    // Initial residual, its projection, the first V-cycle and p: everything
+    // before the first CG iteration (pcg_3d.cpp:150-215), on the bound Ctrl.
+    void start(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol) {
         T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
         {
             TScope ts(timer_, UAAMG_K_INIT);
@@ -531,6 +536,27 @@ public:
         vcycle(r_, z_, skip);
         zsums(skip);
         pupdate();
+    }
+    // p += x (cold start) or p = x (warm start) on fluid cells
+    void finalize_into(double* p_dev, int warm) {
+        if (!x_is_p()) k_finalize<T><<<flat_blocks(2048), 256, 0, stream_>>>(x_, p_dev, nbr0_, g0_.N, warm);
+        HIPCHK(hipGetLastError());
+    }
+    Ctrl* ctrl_ptr() const { return ctrl_; }
+    // mark the solve state idle (done): batched kernels skip this grid
+    void set_idle() {
+        k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, 0, 0.0, o_.tol_mode, project_, o_.flexible, nullptr);
+        HIPCHK(hipGetLastError());
+    }
+
+    int solve(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, uaamg_stats* st) override {
+        auto t0 = std::chrono::steady_clock::now();
+        timer_.on = (o_.timing != 0);
+        timer_.mask = (unsigned)o_.timing;
+        timer_.reset();
+        T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
+        start(rhs_dev, p_dev, warm, max_iter, tol);
+        const int* skip = &ctrl_->done;
         int it_cap = max_iter;
         int every = std::max(1, o_.check_every);
         int queued = 0;

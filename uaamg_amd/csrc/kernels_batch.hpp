@@ -74,6 +74,13 @@ __global__ void kb_ctrl_init(BGrid<S>* G, int ng, int max_iter, double tol, int 
     *G[g].ctrl = c;
 }
 
+// copy every member's CG scalars into one array (one device-to-host copy)
+template <typename S>
+__global__ void kb_gather_ctrl(BGrid<S>* G, int ng, Ctrl* out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ng) out[g] = *G[g].ctrl;
+}
+
 // flat phases: item = (grid, local block, block count)
 template <typename S>
 __global__ __launch_bounds__(256) void kb_init_sum(BGrid<S>* G, const BItem* items, double* partials) {

@@ -163,13 +163,13 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restric
     if (skip && *skip) return;
     __shared__ CT lds[TAIL_LDS_BYTES / sizeof(CT)];
     const int tid = threadIdx.x;
-    const int nl = A.nl;   // 2 <= nl <= NLMAX (checked on the host)
+    const int nl = A.nl;   // 1 <= nl <= NLMAX (checked on the host); 1: the coarsest level only
 
     // registers: the first tail level has up to 2 pairs per thread, the others 1
     TailPair<CT> P0[TAIL_MAXP], P1[1], P2[1], P3[1], P4[1];
 #pragma unroll
     for (int v = 0; v < TAIL_MAXP; v++) tail_load_pair<D, CT, C4>(P0[v], A.coef[0], A.g[0], tid + TAIL_THREADS * v);
-    tail_load_pair<D, CT, C4>(P1[0], A.coef[1], A.g[1], tid);
+    if (nl > 1) tail_load_pair<D, CT, C4>(P1[0], A.coef[1], A.g[1], tid); else P1[0].valid = false;
     if constexpr (NLMAX > 2) { if (nl > 2) tail_load_pair<D, CT, C4>(P2[0], A.coef[2], A.g[2], tid); }
     if constexpr (NLMAX > 3) { if (nl > 3) tail_load_pair<D, CT, C4>(P3[0], A.coef[3], A.g[3], tid); }
     if constexpr (NLMAX > 4) { if (nl > 4) tail_load_pair<D, CT, C4>(P4[0], A.coef[4], A.g[4], tid); }
@@ -217,11 +217,12 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restric
         __syncthreads();
     };
     // down
-    level_down(P0, 0);
+    if (nl > 1) level_down(P0, 0);
     if constexpr (NLMAX > 2) { if (nl > 2) level_down(P1, 1); }
     if constexpr (NLMAX > 3) { if (nl > 3) level_down(P2, 2); }
     if constexpr (NLMAX > 4) { if (nl > 4) level_down(P3, 3); }
     // coarsest
+    if (nl == 1) level_coarsest(P0, 0);
     if (nl == 2) level_coarsest(P1, 1);
     if constexpr (NLMAX > 2) { if (nl == 3) level_coarsest(P2, 2); }
     if constexpr (NLMAX > 3) { if (nl == 4) level_coarsest(P3, 3); }
@@ -230,7 +231,7 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restric
     if constexpr (NLMAX > 4) { if (nl > 4) level_up(P3, 3); }
     if constexpr (NLMAX > 3) { if (nl > 3) level_up(P2, 2); }
     if constexpr (NLMAX > 2) { if (nl > 2) level_up(P1, 1); }
-    level_up(P0, 0);
+    if (nl > 1) level_up(P0, 0);
     {
         const CT* X0 = lds + A.xo[0];
         for (int e = tid; e < A.g[0].N; e += TAIL_THREADS) x_first[e] = (S)X0[e];

@@ -1,6 +1,9 @@
 // uaamg.hip -- C-ABI (include/uaamg.h) over the gfx950 engines.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
+#include <deque>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -485,7 +488,8 @@ struct uaamg_batch {
     std::string err;
     hipStream_t stream = nullptr;
     std::unique_ptr<uaamg::BatchBase> part[2];       // [0]: 2D members, [1]: 3D members
-    std::vector<std::pair<int, int>> members;        // (part, index in part) in order of addition
+    std::vector<std::unique_ptr<EngineBase>> solo;   // members a batch cannot take (a one-level hierarchy)
+    std::vector<std::pair<int, int>> members;        // (part 0/1, index in part) or (2, index in solo)
     uint8_t* stage = nullptr; long stage_n = 0;
 };
 
@@ -566,9 +570,18 @@ static int batch_add_impl(uaamg_batch* b, const uaamg_grid* g, int* id, bool dev
         double h[3] = {g->h[0], g->h[1], g->dim == 3 ? g->h[2] : 1.0};
         int wall[6] = {0, 0, 0, 0, 0, 0};
         for (int f = 0; f < 2 * g->dim; f++) wall[f] = g->wall[f];
-        const int local = b->part[pi]->add(n, h, wall, src);
+        auto shape = hierarchy_shape(g->dim, n[0], n[1], n[2], b->o.coarsest_side);
+        const bool batchable = shape.size() >= 2 &&
+            (g->dim == 3 ? Engine<3, float, float>::choose_tail(shape) : Engine<2, float, float>::choose_tail(shape)) > 0;
+        if (batchable) {
+            const int local = b->part[pi]->add(n, h, wall, src);
+            b->members.emplace_back(pi, local);
+        } else {   // solved on its own inside uaamg_batch_solve
+            b->solo.push_back(make_engine(b->o, *g, b->stream));
+            b->solo.back()->set_grid(src);
+            b->members.emplace_back(2, (int)b->solo.size() - 1);
+        }
         HIPCHK(hipStreamSynchronize(b->stream));
-        b->members.emplace_back(pi, local);
         if (id) *id = (int)b->members.size() - 1;
     })
     return UAAMG_OK;
@@ -581,6 +594,7 @@ extern "C" int uaamg_batch_clear(uaamg_batch* b) {
     if (!b) return UAAMG_ERR_ARG;
     BATCH_TRY(b, {
         for (auto& p : b->part) if (p) p->clear();
+        b->solo.clear();
         b->members.clear();
     })
     return UAAMG_OK;
@@ -592,6 +606,7 @@ extern "C" size_t uaamg_batch_bytes(const uaamg_batch* b) {
     if (!b) return 0;
     size_t s = 0;
     for (auto& p : b->part) if (p) s += p->bytes();
+    for (auto& e : b->solo) s += e->bytes();
     return s;
 }
 
@@ -610,6 +625,7 @@ extern "C" int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, d
         std::vector<int> warm[2];
         for (int i = 0; i < n; i++) {
             const int pi = b->members[i].first;
+            if (pi == 2) continue;
             rhs[pi].push_back(rhs_dev[i]);
             pout[pi].push_back(p_dev[i]);
             warm[pi].push_back(warm_start ? warm_start[i] : 0);
@@ -628,15 +644,288 @@ extern "C" int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, d
         }
         for (auto& p : b->part) if (p && p->size()) p->finish();
         HIPCHK(hipStreamSynchronize(b->stream));
+        std::vector<uaamg_stats> solo_st(b->solo.size());
+        for (int i = 0; i < n; i++)
+            if (b->members[i].first == 2)
+                b->solo[b->members[i].second]->solve(rhs_dev[i], p_dev[i], warm_start ? warm_start[i] : 0, max_iter, tol,
+                                                     &solo_st[b->members[i].second]);
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         int worst = UAAMG_OK;
         for (int i = 0; i < n; i++) {
             uaamg_stats tmp;
-            b->part[b->members[i].first]->stats(b->members[i].second, &tmp, ms);
+            if (b->members[i].first == 2) { tmp = solo_st[b->members[i].second]; tmp.ms_total = ms; }
+            else b->part[b->members[i].first]->stats(b->members[i].second, &tmp, ms);
             if (st) st[i] = tmp;
             if (tmp.status != UAAMG_OK && worst == UAAMG_OK) worst = tmp.status;
         }
         return worst;
     })
+    return UAAMG_OK;
+}
+
+// ------------------------------------------------------------------ serving C-ABI
+//
+// A request loop over batches with continuous admission: each simulation
+// (registered once with its grid) keeps its hierarchy resident on the device
+// while memory allows; requests (one pressure solve of one frame) queue in
+// arrival order; every step admits what fits -- a simulation's next frame once
+// its previous one is done, new hierarchies within the HBM budget, evicting
+// the least recently used idle simulations -- starts the admitted requests,
+// runs one chunk of iterations of every active request as one batch, and
+// completes the converged ones.  Requests that arrive while others run join
+// at the next chunk.
+
+namespace {
+struct SimState {
+    int dim = 0, n[3] = {1, 1, 1}, wall[6] = {0, 0, 0, 0, 0, 0};
+    double h[3] = {1, 1, 1};
+    std::vector<uint8_t> mask;
+    int part = -1, slot = -1;          // resident: batch part and member index
+    size_t bytes = 0;
+    long last_used = 0;
+    bool busy = false;
+};
+struct Request {
+    int sim = -1;
+    const double* rhs = nullptr;
+    double* p = nullptr;
+    int warm = 0, max_iter = 0;
+    double tol = 0;
+    int state = 0;                     // 0 queued, 1 running, 2 done
+    uaamg_stats st{};
+    std::chrono::steady_clock::time_point t_submit, t_done;
+};
+}  // namespace
+
+struct uaamg_server {
+    uaamg_opts o{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, used = 0;
+    int max_active = 0;
+    std::unique_ptr<uaamg::BatchBase> part[2];
+    std::vector<int> slot_sim[2];
+    std::unordered_map<int, SimState> sims;
+    std::vector<Request> reqs;
+    std::deque<int> queue;
+    std::vector<int> active;
+    long tick = 0;
+    uint8_t* stage = nullptr; long stage_n = 0;
+};
+
+static int sfail(uaamg_server* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define SERVER_TRY(s, ...)                                                                         \
+    try { __VA_ARGS__ }                                                                            \
+    catch (const HipError& e) {                                                                    \
+        return sfail(s, e.e == hipErrorOutOfMemory ? UAAMG_ERR_OOM : UAAMG_ERR_HIP,               \
+                     std::string(hipGetErrorString(e.e)) + " (" + e.where + ")");                 \
+    }                                                                                              \
+    catch (const std::bad_alloc&) { return sfail(s, UAAMG_ERR_OOM, "host allocation failed"); }
+
+extern "C" int uaamg_server_create(uaamg_server** out, const uaamg_opts* o, size_t hbm_budget, int max_active) {
+    if (!out || max_active < 1) return UAAMG_ERR_ARG;
+    *out = nullptr;
+    uaamg_opts d;
+    uaamg_default_opts(&d);
+    d.precision = UAAMG_FP32;
+    if (!o) o = &d;
+    std::string why;
+    if (!opts_valid(o, why)) return UAAMG_ERR_ARG;
+    if (o->precision != UAAMG_FP32 && o->precision != UAAMG_MIXED_BF16) return UAAMG_ERR_ARG;
+    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused) return UAAMG_ERR_ARG;
+    int rc = check_device(why);
+    if (rc) { std::fprintf(stderr, "uaamg_server_create: %s\n", why.c_str()); return rc; }
+    uaamg_server* s = new (std::nothrow) uaamg_server();
+    if (!s) return UAAMG_ERR_OOM;
+    s->o = *o;
+    s->budget = hbm_budget;
+    s->max_active = max_active;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { delete s; return UAAMG_ERR_HIP; }
+    *out = s;
+    return UAAMG_OK;
+}
+
+extern "C" void uaamg_server_destroy(uaamg_server* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->part[0].reset();
+    s->part[1].reset();
+    if (s->stage) (void)hipFree(s->stage);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" const char* uaamg_server_last_error(const uaamg_server* s) { return s ? s->err.c_str() : "null server"; }
+
+extern "C" int uaamg_server_register(uaamg_server* s, int sim, const uaamg_grid* g) {
+    if (!s) return UAAMG_ERR_ARG;
+    std::string why;
+    if (!grid_valid(g, why)) return sfail(s, UAAMG_ERR_ARG, why);
+    if (s->sims.count(sim)) return sfail(s, UAAMG_ERR_ARG, "simulation already registered");
+    {
+        auto shape = hierarchy_shape(g->dim, g->n[0], g->n[1], g->dim == 3 ? g->n[2] : 1, s->o.coarsest_side);
+        const bool ok = shape.size() >= 2 &&
+            (g->dim == 3 ? Engine<3, float, float>::choose_tail(shape) : Engine<2, float, float>::choose_tail(shape)) > 0;
+        if (!ok) return sfail(s, UAAMG_ERR_ARG, "grid too small or too flat for the serving batches (needs 2 levels)");
+    }
+    SimState st;
+    st.dim = g->dim;
+    for (int q = 0; q < 3; q++) { st.n[q] = q < g->dim ? g->n[q] : 1; st.h[q] = q < g->dim ? g->h[q] : 1.0; }
+    for (int f = 0; f < 6; f++) st.wall[f] = f < 2 * g->dim ? g->wall[f] : 0;
+    const long N = (long)(st.n[0] + 2) * (st.n[1] + 2) * (g->dim == 3 ? st.n[2] + 2 : 1);
+    st.mask.assign(g->solid, g->solid + N);
+    s->sims.emplace(sim, std::move(st));
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_server_submit(uaamg_server* s, int sim, const double* rhs_dev, double* p_dev, int warm_start,
+                                   int max_iter, double tol, int* req) {
+    if (!s || !rhs_dev || !p_dev) return UAAMG_ERR_ARG;
+    if (!s->sims.count(sim)) return sfail(s, UAAMG_ERR_ARG, "unknown simulation");
+    Request r;
+    r.sim = sim; r.rhs = rhs_dev; r.p = p_dev; r.warm = warm_start;
+    r.max_iter = max_iter < 0 ? s->o.max_iter : max_iter;
+    r.tol = tol < 0 ? s->o.tol : tol;
+    r.t_submit = std::chrono::steady_clock::now();
+    s->reqs.push_back(r);
+    s->queue.push_back((int)s->reqs.size() - 1);
+    if (req) *req = (int)s->reqs.size() - 1;
+    return UAAMG_OK;
+}
+
+static size_t sim_estimate(const uaamg_server* s, const SimState& st) {
+    uaamg_grid g{};
+    g.dim = st.dim;
+    for (int q = 0; q < 3; q++) { g.n[q] = st.n[q]; g.h[q] = st.h[q]; }
+    g.solid = st.mask.data();
+    for (int f = 0; f < 6; f++) g.wall[f] = st.wall[f];
+    return uaamg_bytes_required(&g, &s->o);
+}
+
+// drop the least recently used idle resident simulation; false if none
+static bool server_evict_one(uaamg_server* s) {
+    int victim = -1;
+    long best = 0;
+    for (auto& kv : s->sims) {
+        const SimState& st = kv.second;
+        if (st.slot < 0 || st.busy) continue;
+        if (victim < 0 || st.last_used < best) { victim = kv.first; best = st.last_used; }
+    }
+    if (victim < 0) return false;
+    SimState& v = s->sims[victim];
+    const int pi = v.part, slot = v.slot;
+    s->part[pi]->remove(slot);
+    s->slot_sim[pi].erase(s->slot_sim[pi].begin() + slot);
+    for (int k = slot; k < (int)s->slot_sim[pi].size(); k++) s->sims[s->slot_sim[pi][k]].slot = k;
+    s->used -= std::min(s->used, v.bytes);
+    v.slot = -1; v.part = -1; v.bytes = 0;
+    return true;
+}
+
+extern "C" int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* n_done) {
+    if (!s) return UAAMG_ERR_ARG;
+    if (n_done) *n_done = 0;
+    SERVER_TRY(s, {
+        // admission, in arrival order; new members join their batch before any start
+        std::vector<int> starting;
+        for (auto it = s->queue.begin(); it != s->queue.end() && (int)(s->active.size() + starting.size()) < s->max_active;) {
+            Request& r = s->reqs[*it];
+            SimState& st = s->sims[r.sim];
+            if (st.busy) { ++it; continue; }              // its previous frame is still running
+            if (st.slot < 0) {
+                const size_t need = sim_estimate(s, st);
+                while (s->used + need > s->budget && server_evict_one(s)) {}
+                if (s->used + need > s->budget) { ++it; continue; }   // waits for memory
+                const int pi = st.dim == 3 ? 1 : 0;
+                if (!s->part[pi]) s->part[pi] = make_batch_part(st.dim, s->o, s->stream);
+                const long N = (long)st.mask.size();
+                if (s->stage_n < N) {
+                    if (s->stage) HIPCHK(hipFree(s->stage));
+                    s->stage = nullptr;
+                    HIPCHK(hipMalloc(&s->stage, N));
+                    s->stage_n = N;
+                }
+                HIPCHK(hipMemcpyAsync(s->stage, st.mask.data(), N, hipMemcpyHostToDevice, s->stream));
+                st.slot = s->part[pi]->add(st.n, st.h, st.wall, s->stage);
+                st.part = pi;
+                s->slot_sim[pi].push_back(r.sim);
+                st.bytes = s->part[pi]->member_bytes(st.slot);
+                s->used += st.bytes;
+            }
+            st.busy = true;
+            st.last_used = ++s->tick;
+            r.state = 1;
+            starting.push_back(*it);
+            it = s->queue.erase(it);
+        }
+        for (auto& p : s->part) if (p) p->prepare();
+        for (int id : starting) {
+            Request& r = s->reqs[id];
+            SimState& st = s->sims[r.sim];
+            s->part[st.part]->start_member(st.slot, r.rhs, r.p, r.warm, r.max_iter, r.tol);
+            s->active.push_back(id);
+        }
+        if (s->active.empty()) return UAAMG_OK;
+        // one chunk of iterations for every active request, per batch part
+        bool run[2] = {false, false};
+        for (int id : s->active) run[s->sims[s->reqs[id].sim].part] = true;
+        for (int pi = 0; pi < 2; pi++) if (run[pi]) s->part[pi]->run_chunk();
+        HIPCHK(hipStreamSynchronize(s->stream));
+        // completions
+        std::vector<int> still;
+        int nd = 0;
+        for (int id : s->active) {
+            Request& r = s->reqs[id];
+            SimState& st = s->sims[r.sim];
+            const uaamg::Ctrl& c = s->part[st.part]->member_ctrl(st.slot);
+            if (!c.done) { still.push_back(id); continue; }
+            s->part[st.part]->finish_member(st.slot, r.p, r.warm);
+            r.st.iters = c.iter;
+            r.st.status = c.status == uaamg::ST_CONVERGED ? UAAMG_OK
+                        : c.status == uaamg::ST_BREAKDOWN ? UAAMG_BREAKDOWN
+                        : c.status == uaamg::ST_NONFINITE ? UAAMG_NONFINITE : UAAMG_NOT_CONVERGED;
+            r.st.rnorm = std::sqrt(std::max(0.0, c.rsnew));
+            r.st.r0norm = std::sqrt(std::max(0.0, c.r0sq));
+            r.state = 2;
+            st.busy = false;
+            st.last_used = ++s->tick;
+            if (done && nd < max_done) done[nd] = id;
+            nd++;
+        }
+        s->active.swap(still);
+        if (nd) HIPCHK(hipStreamSynchronize(s->stream));
+        const auto now = std::chrono::steady_clock::now();
+        for (int k = 0; k < nd && done && k < max_done; k++) {
+            Request& r = s->reqs[done[k]];
+            r.t_done = now;
+            r.st.ms_total = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+        }
+        if (n_done) *n_done = std::min(nd, max_done);
+    })
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_server_request(const uaamg_server* s, int req, uaamg_stats* st) {
+    if (!s || req < 0 || req >= (int)s->reqs.size() || !st) return UAAMG_ERR_ARG;
+    const Request& r = s->reqs[req];
+    if (r.state != 2) return UAAMG_NOT_CONVERGED;
+    *st = r.st;
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_server_status(const uaamg_server* s, int* queued, int* active, int* resident, size_t* bytes_used) {
+    if (!s) return UAAMG_ERR_ARG;
+    if (queued) *queued = (int)s->queue.size();
+    if (active) *active = (int)s->active.size();
+    if (resident) {
+        int n = 0;
+        for (auto& kv : s->sims) n += kv.second.slot >= 0;
+        *resident = n;
+    }
+    if (bytes_used) *bytes_used = s->used;
     return UAAMG_OK;
 }
