@@ -148,7 +148,9 @@ def bench_batch(args):
         return dt, outs, iters
 
     def batched(prec, reps):
-        b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000)
+        # flexible CG (Polak-Ribiere beta) absorbs the 16-bit levels' rounding
+        b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000,
+                    flexible=1 if prec == U.MIXED_BF16 else 0)
         for p in pbs:
             b.add(p.n, p.h, p.solid, p.wall)
         outs = [torch.zeros(p.N, dtype=torch.float64, device=dev) for p in pbs]
@@ -290,7 +292,7 @@ def bench_serve(args):
     log(f"[serve] generated {nsim} simulations ({cells / 1e6:.1f} M cells) in {time.perf_counter() - t0:.1f}s")
     budget = int(args.serve_budget_gb * (1 << 30))
     srv = U.Server(budget, max_active=args.serve_slots, precision=U.MIXED_BF16, tol_mode=U.TOL_REL, use_graph=1,
-                   max_iter=500)
+                   max_iter=500, flexible=1)
     for k, s in enumerate(sims):
         srv.register(k, s["n"], s["h"], s["solid"])
     p = [torch.zeros(int(np.prod(s["shape"])), dtype=torch.float64, device=dev) for s in sims]

@@ -1,0 +1,69 @@
+"""GPU: the named workloads beyond the 256^3 benchmark -- the 2D Dirichlet
+Poisson problem (V(2,2), coarsest 8^2, FP64) against the oracle and the
+analytic solution, the standalone amgpcg at the reference's golden sizes, and
+the 2D anisotropic MAC projection (FP32 vs the FP64 path)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import uaamg_amd as U
+from uaamg_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_poisson2d_dirichlet_v22_coarsest8(gpu):
+    """256^2 cell-centred unit square, p = 0 on every wall, V(2,2) cycles
+    coarsened to 8^2, FP64, relres 1e-6: same iterations and pressure as the
+    oracle with the same options; second-order agreement with -sin sin."""
+    pb = W.poisson2d_dirichlet(seed=0, n=256)
+    opb = ob.Problem(pb.n, pb.L, pb.solid, pb.wall)
+    po, res = ob.pcg(opb, pb.rhs, 500, 1e-6, tol_mode=1, nu_pre=2, nu_post=2, coarsest_side=8)
+    c = U.Context(precision=U.FP64, tol_mode=U.TOL_REL, nu_pre=2, nu_post=2, coarsest_side=8)
+    c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+    p, st = c.solve(pb.rhs, 500, 1e-6)
+    c.close()
+    assert st.status == 0 and res.status == 0
+    assert abs(st.iters - res.iters) <= 1, (st.iters, res.iters)
+    m = pb.fluid_mask()
+    assert np.abs(p[m] - po[m]).max() <= 1e-9 * np.abs(po[m]).max()
+    # discretisation error of the 5-point Laplacian with ghost = -interior walls: O(h^2)
+    assert np.abs(p[m] - pb.exact[m]).max() < 2e-4
+
+
+@pytest.mark.parametrize("N", [64, 128, 256])
+def test_amgpcg_golden_iterations(gpu, N):
+    """The reference's own amgpcg iteration counts (test_amgpcg.cpp sin-sin case)."""
+    d = np.load(os.path.join(GOLDEN, f"g4_amgpcg_sin_N{N}.npz"))
+    h = 1.0 / (N + 1)
+    i = np.arange(N + 2)
+    b = 2 * np.pi ** 2 * np.outer(np.sin(np.pi * i * h), np.sin(np.pi * i * h))
+    b[0, :] = b[-1, :] = 0
+    b[:, 0] = b[:, -1] = 0
+    x, it = U.load().amgpcg(N, b.reshape(-1), int(d["max_iter"]), float(d["tol"]), U.FP64)
+    assert it == int(d["iters"]), (it, int(d["iters"]))
+    X = x.reshape(N + 2, N + 2)
+    exact = np.outer(np.sin(np.pi * i * h), np.sin(np.pi * i * h))
+    assert np.abs(X[1:-1, 1:-1] - exact[1:-1, 1:-1]).max() < 1e-2
+
+
+def test_pressure2d_mac_fp32_vs_fp64(gpu):
+    """1024 x 512 MAC projection, dx = 1/1024, dy = 1/480, solid disc, FP32 at
+    relres 1e-6 against the FP64 path (which equals the oracle to 1e-9 on
+    every parity geometry)."""
+    pb = W.pressure2d_mac(seed=0)
+    out = {}
+    for prec in (U.FP64, U.FP32):
+        c = U.Context(precision=prec, tol_mode=U.TOL_REL)
+        c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+        out[prec] = c.solve(pb.rhs, 2000, 1e-6 if prec == U.FP32 else 1e-10)
+        c.close()
+    (p64, s64), (p32, s32) = out[U.FP64], out[U.FP32]
+    assert s64.status == 0 and s32.status == 0
+    m = pb.fluid_mask()
+    a, b = p32[m] - p32[m].mean(), p64[m] - p64[m].mean()
+    assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max()
+    print(f"pressure2d_mac: fp32 {s32.iters} iterations, fp64 (1e-10) {s64.iters}")

@@ -1,12 +1,13 @@
-// kernels_cg.hpp -- device-resident PCG on level 0: FP64 reductions with a
-// deterministic last-block finalisation; alpha/beta/mean/convergence decided
-// on device so the host only polls Ctrl::done every few iterations.
+// kernels_cg.hpp -- device-resident PCG on level 0: FP64 reductions written
+// as one partial per workgroup and summed in block order by one-workgroup
+// finish kernels, which also decide alpha/beta/mean/convergence on device, so
+// the host only polls Ctrl::done every few iterations.
 //
 // The vector kernels run flat over the padded array (4 consecutive cells per
 // thread, 16/32-byte vector loads; cells whose neighbour byte has no fluid bit
-// are left alone) on a persistent grid of a few workgroups per CU, so the
-// agent-scope release of the last-block reduction happens once per resident
-// workgroup, at its end, instead of once per tile.
+// are left alone).  Each kernel body takes its (virtual) block index and
+// block count, so the same body serves a single grid (blockIdx) and one grid's
+// slice of a batched work-list.
 //
 // Reference behaviour (paths relative to the reference root):
 //   src/solver/pcg_3d.cpp:18-125 / pcg.cpp:18-126

@@ -456,13 +456,30 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     if (!o) o = &d;
     auto shape = hierarchy_shape(g->dim, g->n[0], g->n[1], g->dim == 3 ? g->n[2] : 1, o->coarsest_side);
     if (shape.empty()) return 0;
-    size_t sv = (o->precision == UAAMG_FP64) ? 8 : 4;
-    size_t sl = (o->precision == UAAMG_FP64) ? 8 : (o->precision == UAAMG_FP32 ? 4 : 2);
-    size_t N0 = (size_t)shape[0].N;
-    size_t bytes = N0 * (2 + 4 * sv + (o->precision == UAAMG_FP64 ? 0 : sv));
-    for (size_t l = 1; l < shape.size(); l++) bytes += (size_t)shape[l].N * (2 * sl + 1 + 4 * sl);
-    Launch lc = cells_launch(shape[0], g->dim);
-    bytes += (3 * (size_t)lc.nblocks() + 64) * 8 + 4096;
+    // mirrors Engine::allocate (device bytes requested by a context)
+    const size_t sv = (o->precision == UAAMG_FP64) ? 8 : 4;
+    const size_t sl = (o->precision == UAAMG_FP64) ? 8 : (o->precision == UAAMG_FP32 ? 4 : 2);
+    const Geo& g0 = shape[0];
+    const size_t N0 = (size_t)g0.N, N4 = (N0 + 3) / 4 * 4;
+    const size_t margin = (size_t)((g0.sx + 8 + 63) / 64) * 64;
+    size_t bytes = N0 + N4 + 8 + sizeof(uaamg::Ctrl) + 5 * (N4 + 2 * margin) * sv;
+    long maxb = std::max<long>(cells_launch(g0, g->dim).nblocks(), 2048);
+    for (size_t l = 1; l < shape.size(); l++) {
+        bytes += (size_t)shape[l].N * (7 * sl + 1);
+        maxb = std::max(maxb, cells_launch(shape[l], g->dim).nblocks());
+    }
+    // fused level-0 launch (at most) and the row kernels' workgroups
+    {
+        const int TX = (g->dim == 3) ? 60 : 252, TY = (g->dim == 3) ? 12 : 1;
+        const long ta = (g0.nx + TX - 1) / TX, tb = (g->dim == 3) ? (g0.ny + TY - 1) / TY : 1;
+        const int nm = (g->dim == 3) ? g0.nz : g0.ny;
+        maxb = std::max(maxb, ta * tb * std::max(1, (nm + 7) / 8));
+        if (g->dim == 3) {
+            const long tiles = (long)((g0.ny + 15) / 16) * ((g0.nz + 3) / 4);
+            maxb = std::max(maxb, 8 * ((tiles + 7) / 8));
+        }
+    }
+    bytes += (3 * (size_t)maxb + 64) * 8;
     return bytes;
 }
 
