@@ -148,9 +148,7 @@ def bench_batch(args):
         return dt, outs, iters
 
     def batched(prec, reps):
-        # flexible CG (Polak-Ribiere beta) absorbs the 16-bit levels' rounding
-        b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000,
-                    flexible=1 if prec == U.MIXED_BF16 else 0)
+        b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000)
         for p in pbs:
             b.add(p.n, p.h, p.solid, p.wall)
         outs = [torch.zeros(p.N, dtype=torch.float64, device=dev) for p in pbs]

@@ -36,7 +36,7 @@ enum {
 enum {
     UAAMG_FP64 = 0,            /* everything FP64: parity mode */
     UAAMG_FP32 = 1,            /* plain form: FP32 vectors and levels, FP64 reductions/scalars */
-    UAAMG_MIXED_BF16 = 2,      /* FP32 CG; coarse levels stored BF16, FP32 arithmetic */
+    UAAMG_MIXED_BF16 = 2,      /* FP32 CG; 3D coarse levels stored BF16 (FP32 arithmetic); 2D levels stay FP32 */
     UAAMG_MIXED_FP16 = 3       /* FP32 CG; coarse levels stored FP16 (h^2-normalised), FP32 arithmetic */
 };
 
