@@ -4,6 +4,7 @@
 // host waits only when it polls Ctrl::done between batches of iterations.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -83,6 +84,21 @@ struct TScope {
     TScope(Timer& t_, int cls) : t(t_) { t.begin(cls, a); }
     ~TScope() { t.end(a); }
 };
+
+// Launch of the main kernel of a level-0 class.  When the class is timed the
+// events ride on the dispatch itself (hipExtLaunchKernelGGL), so the time is
+// that kernel's execution alone, without event-record gaps or the small
+// finishing kernels that follow it.
+template <typename... KP, typename... Args>
+inline void tlaunch(Timer& t, int cls, void (*kern)(KP...), dim3 grid, dim3 block, hipStream_t s, Args... args) {
+    if (t.want(cls)) {
+        hipEvent_t a = t.get(), b = t.get();
+        hipExtLaunchKernelGGL(kern, grid, block, 0, s, a, b, 0, args...);
+        t.recs.push_back({cls, a, b});
+    } else {
+        kern<<<grid, block, 0, s>>>(args...);
+    }
+}
 
 class EngineBase {
 public:
@@ -271,9 +287,18 @@ public:
         unsigned ta = (unsigned)((g.nx + TX - 1) / TX), tb = (D == 3) ? (unsigned)((g.ny + TY - 1) / TY) : 1u;
         int nm = (D == 3) ? g.nz : g.ny;
         long tiles = (long)ta * tb;
+        static const int kc_min = [] {
+            const char* e = std::getenv("UAAMG_MARCH_KCMIN");
+            return e ? std::max(2, std::atoi(e) / 2 * 2) : 8;
+        }();
+        static const long tmul = [] {
+            const char* e = std::getenv("UAAMG_MARCH_TARGET_X");
+            return e ? std::max(1L, std::atol(e)) : 1L;
+        }();
+        target *= tmul;
         long chunks = std::max<long>(1, target / std::max<long>(1, tiles));
         int KC = (int)((nm + chunks - 1) / chunks);
-        KC = std::max(KC, 8);
+        KC = std::max(KC, kc_min);
         KC += (KC & 1);
         MLaunch m;
         m.KC = KC;
@@ -408,22 +433,14 @@ public:
     }
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            if (rows_cpl_ == 4)
-                k_rows_down0<4, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
-                    r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
-            else
-                k_rows_down0<2, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
-                    r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
+            tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0<4, ROWS_NW, ROWS_RW, S> : k_rows_down0<2, ROWS_NW, ROWS_RW, S>,
+                    dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
         }
     }
     void rows_up0(const T* r, Level& C, T* z, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            if (rows_cpl_ == 4)
-                k_rows_up0<4, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
-                    r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
-            else
-                k_rows_up0<2, ROWS_NW, ROWS_RW, S><<<rows_blocks_, ROWS_NW * 64, 0, stream_>>>(
-                    r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
+            tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? k_rows_up0<4, ROWS_NW, ROWS_RW, S> : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
+                    dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
             k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
         }
     }
@@ -441,13 +458,11 @@ public:
         for (int l = 0; l < lt; l++) {
             Level& C = lv_[l + 1];
             if (l == 0 && rows_cpl_) {
-                TScope ts(timer_, UAAMG_K_DOWN0);
                 rows_down0(r, C, skip);
             } else if (l == 0) {
-                TScope ts(timer_, UAAMG_K_DOWN0);
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
-                k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
-                    r, A0, C.b, C.coef, C.g, m.KC, skip);
+                tlaunch(timer_, UAAMG_K_DOWN0, k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4>, m.grid, dim3(256), stream_,
+                        r, A0, C.b, (const C4*)C.coef, C.g, m.KC, skip);
             } else {
                 TScope ts(timer_, UAAMG_K_COARSE);
                 Level& F = lv_[l];
@@ -477,13 +492,11 @@ public:
         for (int l = lt - 1; l >= 0; l--) {
             Level& C = lv_[l + 1];
             if (l == 0 && rows_cpl_) {
-                TScope ts(timer_, UAAMG_K_UP0);
                 rows_up0(r, C, z, skip);
             } else if (l == 0) {
-                TScope ts(timer_, UAAMG_K_UP0);
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
-                k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S><<<m.grid, 256, 0, stream_>>>(
-                    r, A0, C.x, C.g, z, m.KC, skip, partials_);
+                tlaunch(timer_, UAAMG_K_UP0, k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S>, m.grid, dim3(256), stream_,
+                        r, A0, (const S*)C.x, C.g, z, m.KC, skip, partials_);
                 k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
             } else {
                 TScope ts(timer_, UAAMG_K_COARSE);
@@ -635,20 +648,17 @@ public:
 
     void iteration(T* x, const int* skip) {
         const unsigned fb = flat_blocks(2048);
-        {
-            TScope ts(timer_, UAAMG_K_MATVEC);
-            k_matvec_dot<D, T><<<fb, 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
-            k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
-        }
-        {
-            TScope ts(timer_, UAAMG_K_UPDATE);
-            if (o_.flexible) {
-                k_update<T, true><<<fb, 256, 0, stream_>>>(x, r_, p_, Ap_, z_, nbr0_, N4_, partials_, ctrl_);
-                k_fin_update<true><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
-            } else {
-                k_update<T, false><<<fb, 256, 0, stream_>>>(x, r_, p_, Ap_, z_, nbr0_, N4_, partials_, ctrl_);
-                k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
-            }
+        tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
+                (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+        if (o_.flexible) {
+            tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, true>, dim3(fb), dim3(256), stream_,
+                    x, r_, (const T*)p_, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
+            k_fin_update<true><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+        } else {
+            tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, false>, dim3(fb), dim3(256), stream_,
+                    x, r_, (const T*)p_, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
+            k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
         }
         vcycle(r_, z_, skip);
         zsums(skip);
@@ -663,8 +673,8 @@ public:
         k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
     }
     void pupdate() {
-        TScope ts(timer_, UAAMG_K_PUPDATE);
-        k_pupdate<T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, z_, nbr0_, N4_, ctrl_);
+        tlaunch(timer_, UAAMG_K_PUPDATE, k_pupdate<T>, dim3(flat_blocks(2048)), dim3(256), stream_,
+                p_, (const T*)z_, (const uint8_t*)nbr0_, N4_, ctrl_);
     }
 
     // ---------------- test hooks ----------------

@@ -29,6 +29,9 @@
 namespace uaamg {
 
 template <typename T> using vec4 = T __attribute__((ext_vector_type(4)));
+// four consecutive values at an element-aligned (not vec4-aligned) address:
+// one unaligned vector load on gfx950
+template <typename T> using vec4u = T __attribute__((ext_vector_type(4), aligned(sizeof(T))));
 
 // Grid-stride loop over 4-cell vectors.  fv_b / fv_n are the (virtual)
 // block index and block count: blockIdx.x / gridDim.x for a single grid, or a
@@ -36,6 +39,21 @@ template <typename T> using vec4 = T __attribute__((ext_vector_type(4)));
 #define FLAT_LOOP(N4)                                                                          \
     for (long v_ = (long)fv_b * blockDim.x + threadIdx.x; v_ < (N4) / 4;                      \
          v_ += (long)fv_n * blockDim.x)
+
+// Stencil form: with fv_n a multiple of 8, blocks with fv_b % 8 == x share the
+// contiguous slab x of 8 (block b of a launch runs on XCD b % 8), so the
+// neighbour planes a block reads were mostly fetched into that XCD's L2 by
+// the same XCD; otherwise the plain grid-stride order.  Either way every
+// vec4 index is visited by exactly one thread.
+#define FLAT_LOOP_XCD(N4)                                                                      \
+    const long xv_n_ = (N4) / 4;                                                               \
+    const bool xv_on_ = (fv_n & 7) == 0;                                                       \
+    const long xv_len_ = (xv_n_ + 7) / 8;                                                      \
+    const long xv_beg_ = xv_on_ ? (fv_b & 7) * xv_len_ + (long)(fv_b >> 3) * blockDim.x        \
+                                : (long)fv_b * blockDim.x;                                     \
+    const long xv_end_ = xv_on_ ? min(xv_n_, ((fv_b & 7) + 1) * xv_len_) : xv_n_;              \
+    const long xv_step_ = (long)(xv_on_ ? (fv_n >> 3) : fv_n) * blockDim.x;                    \
+    for (long v_ = xv_beg_ + threadIdx.x; v_ < xv_end_; v_ += xv_step_)
 
 __device__ __forceinline__ uint8_t nbyte(uint32_t w, int q) { return (uint8_t)(w >> (8 * q)); }
 
@@ -194,14 +212,10 @@ __device__ __forceinline__ double apply_A(long id, uint8_t nb, double vc, double
 template <int D, typename T>
 __device__ __forceinline__ void apply_A4(const T* __restrict__ p, long c0, uint32_t w, const vec4<T>& pc,
                                          const Geo& g, const Consts<double>& c, double (&out)[4]) {
-    T pb[4], pt[4], pf[4], pk[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        pb[q] = p[c0 + q - g.sx];
-        pt[q] = p[c0 + q + g.sx];
-        if (D == 3) { pf[q] = p[c0 + q - g.sy]; pk[q] = p[c0 + q + g.sy]; }
-        else { pf[q] = T(0); pk[q] = T(0); }
-    }
+    const vec4<T> pb = *(const vec4u<T>*)(p + c0 - g.sx);
+    const vec4<T> pt = *(const vec4u<T>*)(p + c0 + g.sx);
+    vec4<T> pf = (T)0, pk = (T)0;
+    if (D == 3) { pf = *(const vec4u<T>*)(p + c0 - g.sy); pk = *(const vec4u<T>*)(p + c0 + g.sy); }
     const double left = (double)p[c0 - 1], right = (double)p[c0 + 4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -220,7 +234,7 @@ __device__ __forceinline__ void d_matvec_dot(const T* __restrict__ p, T* __restr
                              Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
     if (ctrl->done) return;
     double v[1] = {0.0};
-    FLAT_LOOP(N4) {
+    FLAT_LOOP_XCD(N4) {
         long c0 = 4 * v_;
         uint32_t w = *(const uint32_t*)(nbr + c0);
         vec4<T> out = (T)0;

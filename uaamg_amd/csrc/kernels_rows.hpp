@@ -53,13 +53,14 @@ struct RowGeo {
     int per_xcd;   // ceil(ntiles / 8)
 };
 
+// Whole-wave shifts by one lane with DPP (wave_shr:1 = 0x138, wave_shl:1 =
+// 0x130; all rows and banks enabled, lanes with no source keep old = 0):
+// one VALU move instead of a ds_bpermute round trip through the LDS unit.
 __device__ __forceinline__ float lane_from_left(float v) {   // lane l gets lane l-1; lane 0 gets 0
-    float u = __shfl_up(v, 1, 64);
-    return (threadIdx.x & 63) == 0 ? 0.0f : u;
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
 }
 __device__ __forceinline__ float lane_from_right(float v) {  // lane l gets lane l+1; lane 63 gets 0
-    float u = __shfl_down(v, 1, 64);
-    return (threadIdx.x & 63) == 63 ? 0.0f : u;
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
 }
 
 // tile of block b of a single-grid launch: consecutive block ids go
@@ -85,6 +86,32 @@ __device__ __forceinline__ float rows_rinv(uint8_t nb, const float* srinv, const
     if (k == 1) d += T.ex[4];
     if (k == g.nz) d += T.ex[5];
     return d >= 1e-30f ? 1.0f / d : 0.0f;
+}
+
+// The CPL values and neighbour bytes of one lane's cells in one vector load
+// each.  Row starts are not 16-byte aligned in the padded layout, so the
+// types carry 4-byte (values) and 1-byte (bytes) alignment; gfx950 global
+// loads take unaligned addresses.  Lanes past nx read the ghost cell and what
+// follows it (inside the array and its margin) and are masked with okx.
+template <int CPL> struct RowVec;
+template <> struct RowVec<4> {
+    typedef float V __attribute__((ext_vector_type(4), aligned(4)));
+    typedef uint32_t B __attribute__((aligned(1)));
+};
+template <> struct RowVec<2> {
+    typedef float V __attribute__((ext_vector_type(2), aligned(4)));
+    typedef uint16_t B __attribute__((aligned(1)));
+};
+template <int CPL>
+__device__ __forceinline__ void row_load(const float* __restrict__ r, const uint8_t* __restrict__ nbr, int id,
+                                         const bool (&okx)[CPL], float (&vr)[CPL], uint8_t (&vn)[CPL]) {
+    const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(r + id);
+    const uint32_t b = *(const typename RowVec<CPL>::B*)(nbr + id);
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        vr[q] = okx[q] ? v[q] : 0.0f;
+        vn[q] = okx[q] ? (uint8_t)(b >> (8 * q)) : (uint8_t)0;
+    }
 }
 
 template <int CPL, int NW, int RW>
@@ -193,15 +220,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         // the compiler wait for exactly the loads a step consumes
         const int po = th.plane(g, P);
 #pragma unroll
-        for (int t = 0; t < RW; t++)
-#pragma unroll
-            for (int q = 0; q < CPL; q++) {
-                const int id = th.roff[t] + po + q;
-                const float rv = r[id];
-                const uint8_t nv = nbr[id];
-                pfr[slot][t][q] = th.okx[q] ? rv : 0.0f;
-                pfn[slot][t][q] = th.okx[q] ? nv : (uint8_t)0;
-            }
+        for (int t = 0; t < RW; t++) row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
     };
     issue(1, th.m0 - 2);
     issue(0, th.m0 - 1);
@@ -366,15 +385,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
         auto issue = [&](int slot, int P) {
             const int po = th.plane(g, P);
 #pragma unroll
-            for (int t = 0; t < RW; t++)
-#pragma unroll
-                for (int q = 0; q < CPL; q++) {
-                    const int id = th.roff[t] + po + q;
-                    const float rv = r[id];
-                    const uint8_t nv = nbr[id];
-                    pfr[slot][t][q] = th.okx[q] ? rv : 0.0f;
-                    pfn[slot][t][q] = th.okx[q] ? nv : (uint8_t)0;
-                }
+            for (int t = 0; t < RW; t++) row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
             // coarse x of the parents; masked where it is used (S1), so the
             // loads stay in flight
             const int kcc = (P + 1) >> 1;
