@@ -88,6 +88,12 @@ def parity_cases():
     pb.solid = balls(pb, [(0.4, 0.1, 0.06)], [0.05])
     out.append(("3d_130x20x12", pb))
     out.append(("2d_40x30_dirichlet", ob.Problem((40, 30), (1.0, 0.75), wall=[ob.DIRICHLET] * 6)))
+    # level 1 wider than 64 cells: the coarse level runs on the record row
+    # kernels (several y tiles and z chunks, walls baked into the records)
+    pb = ob.Problem((200, 70, 40), (1.0, 0.35, 0.2),
+                    wall=[ob.NEUMANN, ob.DIRICHLET, ob.NEUMANN, ob.NEUMANN, ob.DIRICHLET, ob.NEUMANN])
+    pb.solid = random_solids(pb, rng, frac=0.2, nobj=10)
+    out.append(("3d_200x70x40_walls_solids", pb))
     # porous: 40 % random solid cells, many fluid cells with no fluid neighbour
     pb = ob.Problem((20, 18, 16))
     m = np.zeros(pb.shape[::-1], np.uint8)

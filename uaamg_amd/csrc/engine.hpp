@@ -18,6 +18,7 @@
 #include "kernels_march.hpp"
 #include "kernels_rows.hpp"
 #include "kernels_tail.hpp"
+#include "kernels_pmv.hpp"
 #include "kernels_batch.hpp"
 
 namespace uaamg {
@@ -51,6 +52,9 @@ inline Launch cells_launch(const Geo& g, int D) {
     L.kz = kz;
     return L;
 }
+
+// elements allocated past the end of level buffers read by the row kernels
+constexpr long kRowSlack = 64;
 
 // Per-class device timing with events (opts.timing).
 struct Timer {
@@ -180,6 +184,7 @@ public:
         timer_.s = stream_;
         allocate();
         tail_setup();
+        rows_coarse_setup();
     }
     ~Engine() override { destroy_graph(); release(); }
 
@@ -413,26 +418,88 @@ public:
                 rtab_.rinv[m] = d >= 1e-30 ? (float)(1.0 / d) : 0.0f;
             }
             rtab_.idx2 = (float)ix; rtab_.idy2 = (float)iy; rtab_.idz2 = (float)iz;
+            for (int m = 0; m < 64; m++)
+                ptab_.d[m] = ix * (((m >> 0) & 1) + ((m >> 1) & 1)) + iy * (((m >> 2) & 1) + ((m >> 3) & 1))
+                           + iz * (((m >> 4) & 1) + ((m >> 5) & 1));
+            ptab_.idx2 = ix; ptab_.idy2 = iy; ptab_.idz2 = iz;
             const double e[6] = {2 * ix, 2 * ix, 2 * iy, 2 * iy, 2 * iz, 2 * iz};
             rtab_.any_dirichlet = 0;
             for (int f = 0; f < 6; f++) {
                 const bool on = wall[f] == UAAMG_WALL_DIRICHLET;
                 rtab_.ex[f] = on ? (float)e[f] : 0.0f;
+                ptab_.ex[f] = on ? e[f] : 0.0;
                 rtab_.any_dirichlet |= on;
             }
-            constexpr int TY = ROWS_NW * ROWS_RW - 4;
-            int KC = 16;
-            if (const char* kc = std::getenv("UAAMG_ROWS_KC")) KC = std::max(4, std::atoi(kc) / 4 * 4);
-            rgeo_.KC = KC;
-            rgeo_.nty = (n_[1] + TY - 1) / TY;
-            rgeo_.nkc = (n_[2] + KC - 1) / KC;
-            rgeo_.ntiles = rgeo_.nty * rgeo_.nkc;
-            rgeo_.per_xcd = (rgeo_.ntiles + 7) / 8;
+            ptab_.any_dirichlet = rtab_.any_dirichlet;
+            rgeo_ = make_rowgeo(n_[1], n_[2]);
             rows_blocks_ = 8u * (unsigned)rgeo_.per_xcd;
+            const char* pe = std::getenv("UAAMG_PMV");
+            pmv_on_ = !(pe && pe[0] == '0');
+            pgeo_ = make_rowgeo(n_[1], n_[2], PMV_NW * PMV_RW - 2);
+            pmv_blocks_ = 8u * (unsigned)pgeo_.per_xcd;
+        }
+    }
+    // Tiles of the row kernels: TY rows by KC planes, KC a multiple of 4 chosen
+    // so that about 256 tiles (one per CU) cover the level (KC <= 16; more
+    // planes per tile amortise the 4 halo plane steps, more tiles fill the chip)
+    static RowGeo make_rowgeo(int ny, int nz, int TY = ROWS_NW * ROWS_RW - 4) {
+        RowGeo rg{};
+        rg.nty = (ny + TY - 1) / TY;
+        int KC = (int)std::min<long>(16, std::max<long>(4, (long)nz * rg.nty / 256 / 4 * 4));
+        if (const char* kc = std::getenv("UAAMG_ROWS_KC")) KC = std::max(4, std::atoi(kc) / 4 * 4);
+        rg.KC = KC;
+        rg.nkc = (nz + KC - 1) / KC;
+        rg.ntiles = rg.nty * rg.nkc;
+        rg.per_xcd = (rg.ntiles + 7) / 8;
+        return rg;
+    }
+    // record levels on the row kernels: cells per lane per level (0: march kernels)
+    int rcpl_[32] = {};
+    RowGeo rgl_[32] = {};
+    void rows_coarse_setup() {
+        for (int l = 0; l < 32; l++) rcpl_[l] = 0;
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            const char* env = std::getenv("UAAMG_ROWS_COARSE");
+            if (env && env[0] == '0') return;
+            const int lt = tail_first_ ? tail_first_ : nlevels() - 1;
+            for (int l = 1; l < lt && l < 32; l++) {
+                const Geo& g = shape_[l];
+                // two cells per lane (64 < nx <= 128): smaller levels stay on the
+                // march kernels (too few rows to fill the chip), wider ones would
+                // need four cells per lane and more registers than the workgroup has
+                if (g.nx <= 64 || g.nx > 128) continue;
+                rcpl_[l] = 2;
+                rgl_[l] = make_rowgeo(g.ny, g.nz);
+            }
+        }
+    }
+    void rows_downc(int l, const int* skip) {
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            Level& F = lv_[l];
+            Level& C = lv_[l + 1];
+            const RowGeo& rg = rgl_[l];
+            k_rows_downc<2, ROWS_NW, ROWS_RW, S, C4><<<8u * (unsigned)rg.per_xcd, ROWS_NW * 64, 0, stream_>>>(
+                F.b, F.coef, F.g, rg, C.b, C.g, skip);
+        }
+    }
+    void rows_upc(int l, const int* skip) {
+        if constexpr (D == 3 && std::is_same<T, float>::value) {
+            Level& F = lv_[l];
+            Level& C = lv_[l + 1];
+            const RowGeo& rg = rgl_[l];
+            k_rows_upc<2, ROWS_NW, ROWS_RW, S, C4><<<8u * (unsigned)rg.per_xcd, ROWS_NW * 64, 0, stream_>>>(
+                F.b, F.coef, F.g, rg, C.x, C.g, F.x, skip);
         }
     }
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
+            if (fr_old_) {   // r = r_old - alpha Ap written to r; r.r -> convergence
+                tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S> : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
+                        dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
+                        rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const Ctrl*)ctrl_);
+                k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+                return;
+            }
             tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0<4, ROWS_NW, ROWS_RW, S> : k_rows_down0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.b, C.g, skip);
         }
@@ -463,6 +530,9 @@ public:
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
                 tlaunch(timer_, UAAMG_K_DOWN0, k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4>, m.grid, dim3(256), stream_,
                         r, A0, C.b, (const C4*)C.coef, C.g, m.KC, skip);
+            } else if (rcpl_[l]) {
+                TScope ts(timer_, UAAMG_K_COARSE);
+                rows_downc(l, skip);
             } else {
                 TScope ts(timer_, UAAMG_K_COARSE);
                 Level& F = lv_[l];
@@ -498,6 +568,9 @@ public:
                 tlaunch(timer_, UAAMG_K_UP0, k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S>, m.grid, dim3(256), stream_,
                         r, A0, (const S*)C.x, C.g, z, m.KC, skip, partials_);
                 k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
+            } else if (rcpl_[l]) {
+                TScope ts(timer_, UAAMG_K_COARSE);
+                rows_upc(l, skip);
             } else {
                 TScope ts(timer_, UAAMG_K_COARSE);
                 Level& F = lv_[l];
@@ -513,7 +586,9 @@ public:
     // ---------------- solve ----------------
     // Initial residual, its projection, the first V-cycle and p: everything
     // before the first CG iteration (pcg_3d.cpp:150-215), on the bound Ctrl.
-    void start(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol) {
+    // With the fused p update (pmv_on_ and fused_p), the first p = z - mean is
+    // formed by the first iteration's k_rows_pmv instead of k_pupdate.
+    void start(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, bool fused_p = false) {
         T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
         {
             TScope ts(timer_, UAAMG_K_INIT);
@@ -532,6 +607,9 @@ public:
                 else fill_async(x_, N4_, nullptr);
             }
             fill_async(p_, N4_, nullptr);
+            if (fused_p) fill_async(p2_, N4_, nullptr);
+            pcur_ = 0;
+            fr_old_ = nullptr;
             const unsigned fb = flat_blocks(2048);
             k_init_sum<<<fb, 256, 0, stream_>>>(rhs_dev, nbr0_, N4_, partials_, ctrl_);
             k_fin_init_sum<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
@@ -548,7 +626,7 @@ public:
         const int* skip = &ctrl_->done;
         vcycle(r_, z_, skip);
         zsums(skip);
-        pupdate();
+        if (!fused_p) pupdate();
     }
     // p += x (cold start) or p = x (warm start) on fluid cells
     void finalize_into(double* p_dev, int warm) {
@@ -568,7 +646,9 @@ public:
         timer_.mask = (unsigned)o_.timing;
         timer_.reset();
         T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
-        start(rhs_dev, p_dev, warm, max_iter, tol);
+        // the split update needs the fused V-cycle (its level-0 row kernels)
+        fr_ = pmv_on_ && !o_.flexible && o_.fused && o_.nu_pre == 1 && o_.nu_post == 1 && fused_ok_;
+        start(rhs_dev, p_dev, warm, max_iter, tol, pmv_on_);
         const int* skip = &ctrl_->done;
         int it_cap = max_iter;
         int every = std::max(1, o_.check_every);
@@ -583,7 +663,10 @@ public:
             std::memcpy(&hc, ctrl_host_, sizeof(Ctrl));
             if (hc.done || queued >= it_cap) break;
         }
-        if (!x_is_p()) {
+        if (fr_) {   // the last iteration's x update is still pending
+            TScope ts(timer_, UAAMG_K_OTHER);
+            k_finalize_lag<T><<<flat_blocks(2048), 256, 0, stream_>>>(x_, pbuf(pcur_), ctrl_, p_dev, nbr0_, g0_.N, warm);
+        } else if (!x_is_p()) {
             TScope ts(timer_, UAAMG_K_OTHER);
             k_finalize<T><<<flat_blocks(2048), 256, 0, stream_>>>(x_, p_dev, nbr0_, g0_.N, warm);
         }
@@ -621,26 +704,35 @@ public:
     // `chunk` CG iterations; with use_graph the full-size chunk is captured
     // once into a hipGraph and replayed (the graph holds kernel nodes only:
     // the kernels read the device-resident scalars and the done flag)
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t gexec_ = nullptr;
-    int graph_chunk_ = 0;
+    // (with the fused p update the captured sequence depends on which buffer
+    // holds p when it starts: one graph per starting buffer)
+    hipGraph_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    int graph_chunk_[2] = {0, 0};
     void destroy_graph() {
-        if (gexec_) (void)hipGraphExecDestroy(gexec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
-        gexec_ = nullptr; graph_ = nullptr; graph_valid_ = false;
+        for (int k = 0; k < 2; k++) {
+            if (gexec_[k]) (void)hipGraphExecDestroy(gexec_[k]);
+            if (graph_[k]) (void)hipGraphDestroy(graph_[k]);
+            gexec_[k] = nullptr; graph_[k] = nullptr; graph_chunk_[k] = 0;
+        }
+        graph_valid_ = false;
     }
     void run_chunk(T* x, const int* skip, int chunk, int every) {
         if (o_.use_graph && !timer_.on && chunk == every) {
-            if (!graph_valid_ || graph_chunk_ != chunk) {
-                destroy_graph();
+            const int k = pcur_;
+            if (!graph_valid_ || graph_chunk_[k] != chunk) {
+                if (gexec_[k]) { (void)hipGraphExecDestroy(gexec_[k]); gexec_[k] = nullptr; }
+                if (graph_[k]) { (void)hipGraphDestroy(graph_[k]); graph_[k] = nullptr; }
                 HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int c = 0; c < chunk; c++) iteration(x, skip);
-                HIPCHK(hipStreamEndCapture(stream_, &graph_));
-                HIPCHK(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+                HIPCHK(hipStreamEndCapture(stream_, &graph_[k]));
+                HIPCHK(hipGraphInstantiate(&gexec_[k], graph_[k], nullptr, nullptr, 0));
                 graph_valid_ = true;
-                graph_chunk_ = chunk;
+                graph_chunk_[k] = chunk;
+                pcur_ = k;   // capture does not run the iterations
             }
-            HIPCHK(hipGraphLaunch(gexec_, stream_));
+            HIPCHK(hipGraphLaunch(gexec_[k], stream_));
+            if (pmv_on_ && (chunk & 1)) pcur_ ^= 1;
             return;
         }
         for (int c = 0; c < chunk; c++) iteration(x, skip);
@@ -648,21 +740,50 @@ public:
 
     void iteration(T* x, const int* skip) {
         const unsigned fb = flat_blocks(2048);
-        tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
-                (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
-        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+        const T* pv = p_;        // the p of this iteration's operator and update
+        unsigned mb = fb;        // workgroups of the operator launch (its partials)
+        if (pmv_on_) {
+            const T* pold = pbuf(pcur_);
+            T* pnew = pbuf(pcur_ ^ 1);
+            if constexpr (D == 3 && std::is_same<T, float>::value) {
+                if (fr_)
+                    tlaunch(timer_, UAAMG_K_MATVEC, rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, true> : k_rows_pmv<2, PMV_NW, PMV_RW, true>,
+                            dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_, (const float*)pold, (float*)pnew,
+                            (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_, partials_, (const Ctrl*)ctrl_, (float*)x);
+                else
+                    tlaunch(timer_, UAAMG_K_MATVEC, rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, false> : k_rows_pmv<2, PMV_NW, PMV_RW, false>,
+                            dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_, (const float*)pold, (float*)pnew,
+                            (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_, partials_, (const Ctrl*)ctrl_, (float*)nullptr);
+            }
+            pv = pnew;
+            mb = pmv_blocks_;
+        } else {
+            tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
+                    (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+        }
+        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)mb, ctrl_);
+        if (fr_) {   // r update and convergence test inside the level-0 down kernel
+            fr_old_ = rbuf(pcur_);
+            T* rnew = rbuf(pcur_ ^ 1);
+            pcur_ ^= 1;
+            vcycle(rnew, z_, skip);
+            fr_old_ = nullptr;
+            zsums(skip);
+            return;
+        }
+        if (pmv_on_) pcur_ ^= 1;
         if (o_.flexible) {
             tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, true>, dim3(fb), dim3(256), stream_,
-                    x, r_, (const T*)p_, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
+                    x, r_, pv, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
             k_fin_update<true><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
         } else {
             tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, false>, dim3(fb), dim3(256), stream_,
-                    x, r_, (const T*)p_, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
+                    x, r_, pv, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
             k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
         }
         vcycle(r_, z_, skip);
         zsums(skip);
-        pupdate();
+        if (!pmv_on_) pupdate();
     }
 
     void zsums(const int*) {
@@ -785,6 +906,21 @@ private:
     std::vector<Level> lv_;
     uint8_t* solid0_ = nullptr; uint8_t* nbr0_ = nullptr;
     T *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *Ap_ = nullptr, *x_ = nullptr;
+    // fused p update + operator (k_rows_pmv): p alternates between p_ and p2_
+    // With fr_ (not flexible) the CG update is split: x += alpha p moves into
+    // the next k_rows_pmv, r -= alpha Ap into the level-0 down kernel, and r
+    // alternates between r_ and r2_ like p.
+    T* p2_ = nullptr;
+    T* r2_ = nullptr;
+    bool pmv_on_ = false;
+    bool fr_ = false;                    // this solve uses the split update
+    const T* fr_old_ = nullptr;          // r_old of the fused down kernel (nullptr: plain)
+    RowGeo pgeo_{};
+    PmvTab ptab_{};
+    unsigned pmv_blocks_ = 0;
+    int pcur_ = 0;                       // buffer holding the current p and r (0: p_ / r_, 1: p2_ / r2_)
+    T* pbuf(int k) const { return k ? p2_ : p_; }
+    T* rbuf(int k) const { return k ? r2_ : r_; }
     double* partials_ = nullptr; double* count_ = nullptr;
     Ctrl* ctrl_ = nullptr; Ctrl* ctrl_host_ = nullptr;
     size_t bytes_ = 0;

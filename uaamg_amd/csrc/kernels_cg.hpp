@@ -190,10 +190,8 @@ __global__ void k_fin_init_sub(const double* partials, int nb, Ctrl* ctrl) {
 // on Neumann walls and solid neighbours, ghost = -self on Dirichlet walls.
 // vl..vk are the six neighbour values (loaded unconditionally by the caller).
 template <int D>
-__device__ __forceinline__ double apply_A(long id, uint8_t nb, double vc, double vl, double vr, double vb, double vt,
-                                          double vf, double vk, const Geo& g, const Consts<double>& c) {
-    int i = 0, j = 0, k = 0;
-    if (c.any_dirichlet) coords<D>(g, id, i, j, k);
+__device__ __forceinline__ double apply_A_at(int i, int j, int k, uint8_t nb, double vc, double vl, double vr, double vb,
+                                             double vt, double vf, double vk, const Geo& g, const Consts<double>& c) {
     double vL = (nb & NB_XM) ? vl : ((c.any_dirichlet && i == 1 && c.wall[0]) ? -vc : vc);
     double vR = (nb & NB_XP) ? vr : ((c.any_dirichlet && i == g.nx && c.wall[1]) ? -vc : vc);
     double vB = (nb & NB_YM) ? vb : ((c.any_dirichlet && j == 1 && c.wall[2]) ? -vc : vc);
@@ -205,6 +203,13 @@ __device__ __forceinline__ double apply_A(long id, uint8_t nb, double vc, double
         Av = Av - (vF + vK) * c.idz2;
     }
     return Av;
+}
+template <int D>
+__device__ __forceinline__ double apply_A(long id, uint8_t nb, double vc, double vl, double vr, double vb, double vt,
+                                          double vf, double vk, const Geo& g, const Consts<double>& c) {
+    int i = 0, j = 0, k = 0;
+    if (c.any_dirichlet) coords<D>(g, id, i, j, k);
+    return apply_A_at<D>(i, j, k, nb, vc, vl, vr, vb, vt, vf, vk, g, c);
 }
 
 // Operator on a 4-cell vector at c0 (any fluid cell in it is interior, so
@@ -266,8 +271,9 @@ __device__ __forceinline__ void d_fin_matvec(const double* partials, int nbt, in
     if (threadIdx.x == 0) {
         double pAp = tot[0];
         ctrl->pAp = pAp;
-        if (!isfinite(pAp)) { ctrl->done = 1; ctrl->status = ST_NONFINITE; ctrl->iter += 1; }
-        else if (pAp < 1e-15) { ctrl->done = 1; ctrl->status = ST_BREAKDOWN; ctrl->iter += 1; }
+        // alpha = 0 when the solve stops here: no x update is pending
+        if (!isfinite(pAp)) { ctrl->done = 1; ctrl->status = ST_NONFINITE; ctrl->iter += 1; ctrl->alpha = 0.0; }
+        else if (pAp < 1e-15) { ctrl->done = 1; ctrl->status = ST_BREAKDOWN; ctrl->iter += 1; ctrl->alpha = 0.0; }
         else ctrl->alpha = ctrl->rsold / pAp;
     }
 }

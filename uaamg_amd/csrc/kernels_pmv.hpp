@@ -1,0 +1,233 @@
+// kernels_pmv.hpp -- level-0 search direction and operator in one pass
+// (FP32 vectors, 3D).
+//
+// One pass replaces the p update of iteration k-1 and the operator of
+// iteration k (pcg_3d.cpp:233-236 and 219-223):
+//   p_new = (z - mean_z) + beta p_old      on fluid cells (0 elsewhere)
+//   Ap    = d p_new - sum_nb c p_new_nb     (FP64 arithmetic on the FP32
+//                                            values, rounded once; d from the
+//                                            neighbour byte + Dirichlet excess)
+//   partial sums of p_new . Ap             (deterministic, per workgroup)
+// and, with XL, the x update of the previous iteration, x += alpha p_old
+// (k_update's formula; alpha = 0 before the first operator), so the CG
+// update itself only has to touch r (fused into the level-0 down kernel).
+// p_old and p_new are separate buffers: a workgroup recomputes p_new on its
+// halo row and planes from z and p_old, so no workgroup reads what another
+// one writes.  p_new is k_pupdate's formula; the operator is k_matvec_dot's
+// (apply_A) in the stencil form of the row kernels: p_new is 0 on every
+// non-fluid cell, so a face to a solid cell or a Neumann wall drops out of
+// both d and the sum, and a Dirichlet wall face adds 2/h^2 to d (apply_A's
+// ghost value -vc).  The operator stays in FP64 like apply_A: for smooth p,
+// d p and the neighbour sum nearly cancel, and an FP32 difference would lose
+// the digits CG needs at relres 1e-6.
+//
+// Mapping as in the row kernels: a wave owns whole x-rows (CPL cells per
+// lane), NW waves of RW rows per workgroup with one halo row on each side,
+// a march over KC planes (+1 halo plane on each side) with the three planes
+// of the stencil in registers, x-neighbours by DPP, y-neighbours of other
+// waves through LDS (one barrier per plane).
+#pragma once
+#include "kernels_cg.hpp"
+#include "kernels_rows.hpp"
+
+namespace uaamg {
+
+constexpr int PMV_NW = 9, PMV_RW = 2;   // 18 rows: 16 output rows and one halo row on each side
+
+// FP64 stencil constants: d per neighbour pattern (nb >> 1), 1/h^2 per axis,
+// Dirichlet excess per face
+struct PmvTab {
+    double d[64];
+    double idx2, idy2, idz2;
+    double ex[6];
+    int any_dirichlet;
+};
+
+template <int CPL, int NW>
+struct PmvLDS {
+    float ex[2][NW][2][CPL][64];   // [step parity][wave][first / last row][cell][lane]
+};
+
+template <int CPL, int NW, int RW, bool XL>
+__device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
+                                           float* __restrict__ pnew, float* __restrict__ Ap,
+                                           const uint8_t* __restrict__ nbr, const Geo& g, const PmvTab& K,
+                                           const RowGeo& rg, double* partials, const Ctrl* ctrl, int tile,
+                                           float* __restrict__ x) {
+    if (ctrl->done) return;
+    __shared__ PmvLDS<CPL, NW> L;
+    __shared__ double sd[64];
+    for (int e = threadIdx.x; e < 64; e += NW * 64) sd[e] = K.d[e];
+    __syncthreads();
+    double acc[1] = {0.0};
+    const bool active = tile < rg.ntiles;   // idle workgroups still store a zero partial
+    const int ty = tile % rg.nty, kc = tile / rg.nty;
+    RowThread<CPL, NW, RW, 1> th;
+    th.init(g, rg, active ? ty : 0, active ? kc : 0);
+    const int lane = th.lane, w = th.w;
+    const double mean = ctrl->mean_z, beta = ctrl->beta, alpha = XL ? ctrl->alpha : 0.0;
+
+    if (active) {
+        float PN[4][RW][CPL];        // p_new of planes (slot = plane & 3)
+        uint8_t NB[4][RW][CPL];
+        float pfz[2][RW][CPL], pfp[2][RW][CPL], pfx[XL ? 2 : 1][RW][CPL];
+        uint8_t pfn[2][RW][CPL];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < RW; t++)
+#pragma unroll
+                for (int q = 0; q < CPL; q++) { PN[s][t][q] = 0.0f; NB[s][t][q] = 0; }
+
+        auto issue = [&](int slot, int P) {
+            const int po = th.plane(g, P);
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                row_load<CPL>(z, nbr, th.roff[t] + po, th.okx, pfz[slot][t], pfn[slot][t]);
+                const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
+#pragma unroll
+                for (int q = 0; q < CPL; q++) pfp[slot][t][q] = v[q];
+                if constexpr (XL) {
+                    const typename RowVec<CPL>::V xv = *(const typename RowVec<CPL>::V*)(x + th.roff[t] + po);
+#pragma unroll
+                    for (int q = 0; q < CPL; q++) pfx[slot][t][q] = xv[q];
+                }
+            }
+        };
+        issue(0, th.m0 - 1);
+        issue(1, th.m0);
+
+        // step: plane P enters (slot S); the operator on plane P-1
+        auto step = [&](auto Sc, int P) {
+            constexpr int S = decltype(Sc)::value;
+            constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3;   // planes P-1, P-2
+            constexpr int PB = S & 1;
+            const bool pin = P >= 1 && P <= g.nz;
+#pragma unroll
+            for (int t = 0; t < RW; t++)
+#pragma unroll
+                for (int q = 0; q < CPL; q++) {
+                    const uint8_t nb = pin ? pfn[PB][t][q] : (uint8_t)0;
+                    NB[S][t][q] = nb;
+                    PN[S][t][q] = (nb & NB_FLUID)
+                        ? (float)(((double)pfz[PB][t][q] - mean) + beta * (double)pfp[PB][t][q]) : 0.0f;
+                }
+            if constexpr (XL) {   // x += alpha p_old on the tile's cells of plane P
+                if (P >= th.m0 && P <= th.last) {
+#pragma unroll
+                    for (int t = 0; t < RW; t++) {
+                        if (!th.tile_row[t]) continue;
+                        float xn[CPL];
+#pragma unroll
+                        for (int q = 0; q < CPL; q++)
+                            xn[q] = (NB[S][t][q] & NB_FLUID) ? (float)((double)pfx[PB][t][q] + alpha * (double)pfp[PB][t][q])
+                                                             : pfx[PB][t][q];
+                        const int id = th.roff[t] + P * (int)g.sy;
+                        if (th.okx[CPL - 1]) {
+                            typename RowVec<CPL>::V v;
+#pragma unroll
+                            for (int q = 0; q < CPL; q++) v[q] = xn[q];
+                            *(typename RowVec<CPL>::V*)(x + id) = v;
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < CPL; q++)
+                                if (th.okx[q]) x[id + q] = xn[q];
+                        }
+                    }
+                }
+            }
+            issue(PB, P + 2);
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                L.ex[PB][w][0][q][lane] = PN[S][0][q];
+                L.ex[PB][w][1][q][lane] = PN[S][RW - 1][q];
+            }
+            const int R = P - 1;
+            if (R >= th.m0 && R <= th.last) {
+                constexpr int RB = S1 & 1;   // LDS buffer written when plane R entered
+                sfor<RW>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    const float el = lane_from_left(PN[S1][t][CPL - 1]);
+                    const float er = lane_from_right(PN[S1][t][0]);
+                    float out[CPL];
+                    sfor<CPL>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        const float vc = PN[S1][t][q];
+                        const float vl = (q > 0) ? PN[S1][t][q > 0 ? q - 1 : 0] : el;
+                        const float vr = (q < CPL - 1) ? PN[S1][t][q < CPL - 1 ? q + 1 : 0] : er;
+                        float vb, vt;
+                        if constexpr (t > 0) vb = PN[S1][t - 1][q]; else vb = L.ex[RB][w > 0 ? w - 1 : 0][1][q][lane];
+                        if constexpr (t < RW - 1) vt = PN[S1][t + 1][q]; else vt = L.ex[RB][w < NW - 1 ? w + 1 : NW - 1][0][q][lane];
+                        const uint8_t nb = NB[S1][t][q];
+                        double d = sd[nb >> 1];
+                        if (K.any_dirichlet) {
+                            const int i = th.i0 + q, j = th.jrow[t];
+                            if (i == 1) d += K.ex[0];
+                            if (i == g.nx) d += K.ex[1];
+                            if (j == 1) d += K.ex[2];
+                            if (j == g.ny) d += K.ex[3];
+                            if (R == 1) d += K.ex[4];
+                            if (R == g.nz) d += K.ex[5];
+                        }
+                        const double sum = K.idx2 * ((double)vl + (double)vr) + K.idy2 * ((double)vb + (double)vt)
+                                         + K.idz2 * ((double)PN[S2][t][q] + (double)PN[S][t][q]);
+                        const bool fl = nb & NB_FLUID;
+                        const float o = fl ? (float)(d * (double)vc - sum) : 0.0f;
+                        acc[0] += (fl && th.tile_row[t] && th.okx[q]) ? (double)vc * (double)o : 0.0;
+                        out[q] = o;
+                    });
+                    if (th.tile_row[t]) {
+                        const int id = th.roff[t] + R * (int)g.sy;
+                        if (th.okx[CPL - 1]) {
+                            typename RowVec<CPL>::V vp, va;
+#pragma unroll
+                            for (int q = 0; q < CPL; q++) { vp[q] = PN[S1][t][q]; va[q] = out[q]; }
+                            *(typename RowVec<CPL>::V*)(pnew + id) = vp;
+                            *(typename RowVec<CPL>::V*)(Ap + id) = va;
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < CPL; q++)
+                                if (th.okx[q]) { pnew[id + q] = PN[S1][t][q]; Ap[id + q] = out[q]; }
+                        }
+                    }
+                });
+            }
+            __syncthreads();
+        };
+        for (int P = th.m0 - 1; P <= th.last + 1; P += 4) {   // m0 - 1 = 0 (mod 4)
+            step(IC<0>{}, P);
+            if (P + 1 > th.last + 1) break;
+            step(IC<1>{}, P + 1);
+            if (P + 2 > th.last + 1) break;
+            step(IC<2>{}, P + 2);
+            if (P + 3 > th.last + 1) break;
+            step(IC<3>{}, P + 3);
+        }
+    }
+    block_store<1>(acc, partials);
+}
+
+template <int CPL, int NW, int RW, bool XL>
+__global__ __launch_bounds__(NW * 64) void k_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
+                                                      float* __restrict__ pnew, float* __restrict__ Ap,
+                                                      const uint8_t* __restrict__ nbr, Geo g, PmvTab K,
+                                                      RowGeo rg, double* partials, const Ctrl* ctrl, float* __restrict__ x) {
+    d_rows_pmv<CPL, NW, RW, XL>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x), x);
+}
+
+// x += alpha p on fluid cells (the x update still pending when the solve
+// stops), then p_out += x (cold start) or p_out = x (warm start), with
+// k_update's and k_finalize's roundings.
+template <typename T>
+__global__ void k_finalize_lag(const T* __restrict__ x, const T* __restrict__ p, const Ctrl* ctrl,
+                               double* __restrict__ pout, const uint8_t* __restrict__ nbr, long N, int assign) {
+    const double alpha = ctrl->alpha;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x, s = (long)gridDim.x * blockDim.x;
+    for (; t < N; t += s)
+        if (nbr[t] & NB_FLUID) {
+            const double xv = (double)(T)((double)x[t] + alpha * (double)p[t]);
+            pout[t] = assign ? xv : pout[t] + xv;
+        }
+}
+
+}  // namespace uaamg

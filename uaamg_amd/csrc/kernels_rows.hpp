@@ -146,13 +146,13 @@ __device__ __forceinline__ void sfor(F&& f) {
     }
 }
 
-// Shared set-up of both kernels.  Addresses are 32-bit element offsets: a
-// row's cells are roff[t] + q.  Rows and planes outside the grid are clamped
-// to the ghost layer (neighbour byte 0: read as non-fluid), lanes past nx are
-// masked with okx.
-template <int CPL, int NW, int RW>
+// Shared set-up of the row kernels (H halo rows on each side of the tile).
+// Addresses are 32-bit element offsets: a row's cells are roff[t] + q.  Rows
+// and planes outside the grid are clamped to the ghost layer (neighbour byte
+// 0: read as non-fluid), lanes past nx are masked with okx.
+template <int CPL, int NW, int RW, int H = 2>
 struct RowThread {
-    static constexpr int TY = NW * RW - 4;
+    static constexpr int TY = NW * RW - 2 * H;
     int lane, w, i0, b0, m0, last;
     int jrow[RW];
     int roff[RW];
@@ -169,8 +169,8 @@ struct RowThread {
         for (int q = 0; q < CPL; q++) okx[q] = i0 + q <= g.nx;
 #pragma unroll
         for (int t = 0; t < RW; t++) {
-            jrow[t] = b0 - 2 + w * RW + t;
-            tile_row[t] = (w * RW + t >= 2) && (w * RW + t < TY + 2) && jrow[t] <= g.ny;
+            jrow[t] = b0 - H + w * RW + t;
+            tile_row[t] = (w * RW + t >= H) && (w * RW + t < TY + H) && jrow[t] <= g.ny;
             const int jc = min(max(jrow[t], 0), g.ny + 1);
             roff[t] = min(i0, g.nx + 1) + jc * (int)g.sx;
         }
@@ -184,15 +184,33 @@ struct RowThread {
 // ------------------------------------------------------------------ down
 // r -> coarse b (restricted residual after the forward sweep).  A workgroup
 // writes the coarse cells whose 8 children lie in its tile.
-template <int CPL, int NW, int RW, typename VC>
+//
+// With FR (fused CG residual update, pcg_3d.cpp:225-231 for r) the kernel
+// reads r_old and Ap and works on r = r_old - alpha Ap (k_update's formula,
+// recomputed on the halo), writes that r on its tile to r_new (another
+// buffer: neighbouring workgroups still read r_old) and stores the partial
+// sums of r.r over its tile's fluid cells.
+struct RUpd {
+    const float* Ap = nullptr;
+    float* rnew = nullptr;
+    double* partials = nullptr;
+    const Ctrl* ctrl = nullptr;
+};
+
+template <int CPL, int NW, int RW, typename VC, bool FR = false>
 __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
                                              const RowTab& K, const RowGeo& rg, VC* __restrict__ bc, const Geo& cg,
-                                             const int* skip, int tile) {
+                                             const int* skip, int tile, const RUpd& u = RUpd{}) {
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
     if (skip && *skip) return;
     __shared__ RowLDS<CPL, NW, RW> L;
-    if (tile >= rg.ntiles) return;   // the whole workgroup leaves together
+    if (tile >= rg.ntiles) {   // the whole workgroup leaves together
+        if constexpr (FR) { double z0[1] = {0.0}; block_store<1>(z0, u.partials); }
+        return;
+    }
     const int ty = tile % rg.nty, kc = tile / rg.nty;
+    const double alpha = FR ? u.ctrl->alpha : 0.0;
+    double rr[1] = {0.0};
     RowThread<CPL, NW, RW> th;
     th.init(g, rg, ty, kc);
     const int lane = th.lane, w = th.w;
@@ -202,6 +220,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
 
     float X[4][RW][CPL], Rq[2][RW][CPL], Iq[4][RW][CPL];
     float pfr[2][RW][CPL];
+    float pfa[FR ? 2 : 1][RW][CPL];
     uint8_t pfn[2][RW][CPL];
     float acc[RW / 2][CPL / 2];
 #pragma unroll
@@ -220,7 +239,14 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         // the compiler wait for exactly the loads a step consumes
         const int po = th.plane(g, P);
 #pragma unroll
-        for (int t = 0; t < RW; t++) row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
+        for (int t = 0; t < RW; t++) {
+            row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
+            if constexpr (FR) {
+                const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(u.Ap + th.roff[t] + po);
+#pragma unroll
+                for (int q = 0; q < CPL; q++) pfa[slot][t][q] = v[q];
+            }
+        }
     };
     issue(1, th.m0 - 2);
     issue(0, th.m0 - 1);
@@ -234,6 +260,32 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         constexpr int ER = EW ^ 1;       // LDS buffer written in the previous step
         // S1: plane P enters; odd cells x0 = r / d.  Parity of (i,j,k) is
         // (q + t + P) & 1 (i0 and the first region row are odd, RW even).
+        if constexpr (FR) {   // r = r_old - alpha Ap; r_new and r.r on the tile
+            const bool pin = P >= th.m0 && P <= th.last;
+            sfor<RW>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const bool fl = pfn[PB][t][q] & NB_FLUID;
+                    const float rn = fl ? (float)((double)pfr[PB][t][q] - alpha * (double)pfa[PB][t][q]) : 0.0f;
+                    pfr[PB][t][q] = rn;
+                    rr[0] += (fl && pin && th.tile_row[t] && th.okx[q]) ? (double)rn * (double)rn : 0.0;
+                });
+                if (pin && th.tile_row[t]) {
+                    const int id = th.roff[t] + P * (int)g.sy;
+                    if (th.okx[CPL - 1]) {
+                        typename RowVec<CPL>::V v;
+#pragma unroll
+                        for (int q = 0; q < CPL; q++) v[q] = pfr[PB][t][q];
+                        *(typename RowVec<CPL>::V*)(u.rnew + id) = v;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < CPL; q++)
+                            if (th.okx[q]) u.rnew[id + q] = pfr[PB][t][q];
+                    }
+                }
+            });
+        }
         sfor<RW>([&](auto tc) {
             constexpr int t = decltype(tc)::value;
             sfor<CPL>([&](auto qc) {
@@ -330,6 +382,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         if (P + 3 > th.last + 2) break;
         step(IC<2>{}, P + 3);
     }
+    if constexpr (FR) block_store<1>(rr, u.partials);
 }
 
 template <int CPL, int NW, int RW, typename VC>
@@ -337,6 +390,17 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
                                                         Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
                                                         const int* skip) {
     d_rows_down0<CPL, NW, RW, VC>(r, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x));
+}
+
+// down0 with the fused residual update (r_old -> r_new, r.r partials)
+template <int CPL, int NW, int RW, typename VC>
+__global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict__ rold, const uint8_t* __restrict__ nbr,
+                                                         Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
+                                                         const int* skip, const float* __restrict__ Ap,
+                                                         float* __restrict__ rnew, double* partials, const Ctrl* ctrl) {
+    RUpd u;
+    u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.ctrl = ctrl;
+    d_rows_down0<CPL, NW, RW, VC, true>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
 }
 
 // ------------------------------------------------------------------ up
@@ -498,6 +562,377 @@ __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ 
                                                       Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
                                                       float* __restrict__ z, const int* skip, double* partials) {
     d_rows_up0<CPL, NW, RW, VCX>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
+}
+
+// ================================================================== record levels
+// Levels l >= 1 (3D) with the same schedule: per-cell records {cx, cy, cz, d}
+// (couplings to the +x/+y/+z neighbours; d < 0: inactive), level values b / x
+// in SV (float, or __bf16 for the BF16 level storage), arithmetic in FP32.
+// The coupling to a -x/-y/-z neighbour is that neighbour's own +coupling:
+// -x from the thread's previous cell or, for cell 0, the previous lane (DPP);
+// -y from the thread's previous row or, for a wave's first row, a load of cy
+// one row up (an L1/L2 hit: the wave above loads that row); -z from the
+// previous plane's slot.  1/d is the hardware reciprocal, as in the march
+// kernels of these levels.  Cells outside the grid and lanes past nx read as
+// inactive.
+
+template <int CPL, typename SV>
+__device__ __forceinline__ void vals_load(const SV* __restrict__ p, int id, const bool (&ok)[CPL], float (&v)[CPL]) {
+    if constexpr (std::is_same<SV, float>::value) {
+        const typename RowVec<CPL>::V u = *(const typename RowVec<CPL>::V*)(p + id);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) v[q] = ok[q] ? u[q] : 0.0f;
+    } else {
+        static_assert(sizeof(SV) == 2, "16-bit level values");
+        typedef unsigned short H __attribute__((ext_vector_type(CPL), aligned(2)));
+        const H u = *(const H*)((const unsigned short*)p + id);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) v[q] = ok[q] ? __builtin_bit_cast(float, (uint32_t)u[q] << 16) : 0.0f;
+    }
+}
+
+template <typename C4> struct RecE;
+template <> struct RecE<float4> { using E = float; };
+template <> struct RecE<BF16x4> { using E = __bf16; };
+
+template <int CPL, int RW, typename C4>
+struct RecPf {            // one plane's loads
+    float b[RW][CPL];
+    C4 rec[RW][CPL];
+};
+
+// LDS of the record kernels: the row exchange, and the cy of each wave's last
+// row per plane slot (the -y coupling of the next wave's first row)
+template <int CPL, int NW, int RW>
+struct RecLDS {
+    RowLDS<CPL, NW, RW> r;
+    float cy[4][NW][CPL][64];
+};
+
+template <int CPL, int NW, int RW>
+struct RecThread : RowThread<CPL, NW, RW> {
+    bool rowok[RW];
+    __device__ void init_rec(const Geo& g, const RowGeo& rg, int ty, int kc) {
+        this->init(g, rg, ty, kc);
+#pragma unroll
+        for (int t = 0; t < RW; t++) rowok[t] = this->jrow[t] >= 1 && this->jrow[t] <= g.ny;
+    }
+};
+
+template <int CPL, int NW, int RW, typename SV, typename C4>
+__device__ __forceinline__ void rec_issue(RecPf<CPL, RW, C4>& f, const SV* __restrict__ b, const C4* __restrict__ rec,
+                                          const RecThread<CPL, NW, RW>& th, const Geo& g, int P) {
+    const int po = th.plane(g, P);
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        vals_load<CPL, SV>(b, th.roff[t] + po, th.okx, f.b[t]);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) f.rec[t][q] = rec[th.roff[t] + po + q];
+    }
+}
+
+// per-plane register state of the record kernels
+template <int CPL, int RW>
+struct RecQ {
+    float X[4][RW][CPL], Iq[4][RW][CPL], Rq[4][RW][CPL];
+    float Cx[4][RW][CPL], Cy[4][RW][CPL], Cz[4][RW][CPL];
+};
+
+// plane P (slot S) enters: unpack the prefetched loads; the last row's cy
+// goes to the LDS slot S for the wave below
+template <int S, int CPL, int NW, int RW, typename C4>
+__device__ __forceinline__ void rec_enter(RecQ<CPL, RW>& Q, const RecPf<CPL, RW, C4>& f, const RecThread<CPL, NW, RW>& th,
+                                          bool planeok, RecLDS<CPL, NW, RW>& L) {
+#pragma unroll
+    for (int t = 0; t < RW; t++)
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const bool ok = planeok && th.rowok[t] && th.okx[q];
+            const C4 r = f.rec[t][q];
+            const float d = ok ? (float)r.w : -1.0f;
+            Q.Cx[S][t][q] = ok ? (float)r.x : 0.0f;
+            Q.Cy[S][t][q] = ok ? (float)r.y : 0.0f;
+            Q.Cz[S][t][q] = ok ? (float)r.z : 0.0f;
+            Q.Iq[S][t][q] = d >= 1e-30f ? __builtin_amdgcn_rcpf(d) : 0.0f;
+            Q.Rq[S][t][q] = d >= 0.0f ? f.b[t][q] : 0.0f;
+        }
+#pragma unroll
+    for (int q = 0; q < CPL; q++) L.cy[S][th.w][q][th.lane] = Q.Cy[S][RW - 1][q];
+}
+
+// sum over the 6 neighbours of cell (T, Qc) of slot s (z+: slot sp, z-: slot
+// sm); edge / cedge: the neighbour value / its coupling across the lane boundary
+// (wave 0's first row is the outer halo row: its -y terms are never output)
+template <int CPL, int NW, int RW, int T, int Qc, int s, int sp, int sm>
+__device__ __forceinline__ float rec_nsum(const RecQ<CPL, RW>& Q, float edge, float cedge, const RecLDS<CPL, NW, RW>& L,
+                                          int eb, int kind, int w, int lane) {
+    float xl, xr, xu, xd, cl, cu;
+    if constexpr (Qc > 0) { xl = Q.X[s][T][Qc - 1]; cl = Q.Cx[s][T][Qc - 1]; } else { xl = edge; cl = cedge; }
+    if constexpr (Qc < CPL - 1) xr = Q.X[s][T][Qc + 1]; else xr = edge;
+    if constexpr (T > 0) { xu = Q.X[s][T - 1][Qc]; cu = Q.Cy[s][T - 1][Qc]; }
+    else { xu = L.r.ex[eb][kind][w > 0 ? w - 1 : 0][1][Qc / 2][lane]; cu = L.cy[s][w > 0 ? w - 1 : 0][Qc][lane]; }
+    if constexpr (T < RW - 1) xd = Q.X[s][T + 1][Qc]; else xd = L.r.ex[eb][kind][w < NW - 1 ? w + 1 : NW - 1][0][Qc / 2][lane];
+    return Q.Cx[s][T][Qc] * xr + cl * xl + Q.Cy[s][T][Qc] * xd + cu * xu + Q.Cz[s][T][Qc] * Q.X[sp][T][Qc]
+         + Q.Cz[sm][T][Qc] * Q.X[sm][T][Qc];
+}
+
+// one colour of plane slot s: x = (b + sum) / d on the cells of parity PAR
+// with 1/d != 0 (others keep x); neighbour rows of other waves from LDS
+// buffer er / kind rk; with PUB, the wave's first / last row go to buffer ew /
+// kind 1 for the next colour
+template <int PAR, bool PUB, int CPL, int NW, int RW, int s, int sp, int sm>
+__device__ __forceinline__ void rec_colour(RecQ<CPL, RW>& Q, RecLDS<CPL, NW, RW>& L, int er, int ew, int rk, int w,
+                                           int lane) {
+    sfor<RW>([&](auto tc) {
+        constexpr int t = decltype(tc)::value;
+        constexpr bool q0 = ((t + s) & 1) == PAR;   // cell 0 of this row has parity PAR
+        const float edge = q0 ? lane_from_left(Q.X[s][t][CPL - 1]) : lane_from_right(Q.X[s][t][0]);
+        const float cedge = q0 ? lane_from_left(Q.Cx[s][t][CPL - 1]) : 0.0f;
+        sfor<CPL>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (((q + t + s) & 1) == PAR) {
+                const float sum = rec_nsum<CPL, NW, RW, t, q, s, sp, sm>(Q, edge, cedge, L, er, rk, w, lane);
+                const float ri = Q.Iq[s][t][q];
+                Q.X[s][t][q] = (Q.Rq[s][t][q] + sum) * ri + ((ri != 0.0f) ? 0.0f : Q.X[s][t][q]);
+            }
+        });
+    });
+    if constexpr (PUB) {
+        sfor<CPL>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (((q + s) & 1) == PAR) L.r.ex[ew][1][w][0][q / 2][lane] = Q.X[s][0][q];
+            if constexpr (((q + RW - 1 + s) & 1) == PAR) L.r.ex[ew][1][w][1][q / 2][lane] = Q.X[s][RW - 1][q];
+        });
+    }
+}
+
+// b (level l) -> coarse b: forward sweep and restricted residual
+template <int CPL, int NW, int RW, typename SV, typename C4>
+__device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4* __restrict__ rec, const Geo& g,
+                                             const RowGeo& rg, SV* __restrict__ bc, const Geo& cg, const int* skip, int tile) {
+    static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
+    if (skip && *skip) return;
+    __shared__ RecLDS<CPL, NW, RW> L;
+    if (tile >= rg.ntiles) return;
+    const int ty = tile % rg.nty, kc = tile / rg.nty;
+    RecThread<CPL, NW, RW> th;
+    th.init_rec(g, rg, ty, kc);
+    const int lane = th.lane, w = th.w;
+    for (int e = threadIdx.x; e < 2 * 2 * NW * 2 * (CPL / 2) * 64; e += NW * 64) (&L.r.ex[0][0][0][0][0][0])[e] = 0.0f;
+    for (int e = threadIdx.x; e < 4 * NW * CPL * 64; e += NW * 64) (&L.cy[0][0][0][0])[e] = 0.0f;
+    __syncthreads();
+
+    RecQ<CPL, RW> Q;
+    RecPf<CPL, RW, C4> pf[2];
+    float acc[RW / 2][CPL / 2];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                Q.X[s4][t][q] = 0.0f; Q.Iq[s4][t][q] = 0.0f; Q.Rq[s4][t][q] = 0.0f;
+                Q.Cx[s4][t][q] = 0.0f; Q.Cy[s4][t][q] = 0.0f; Q.Cz[s4][t][q] = 0.0f;
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+        for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
+    rec_issue<CPL, NW, RW, SV, C4>(pf[1], b, rec, th, g, th.m0 - 2);
+    rec_issue<CPL, NW, RW, SV, C4>(pf[0], b, rec, th, g, th.m0 - 1);
+
+    auto step = [&](auto Sc, int P) {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;
+        constexpr int PB = S & 1, EW = S & 1, ER = EW ^ 1;
+        // S1: plane P enters; odd cells x0 = b / d; isolated cells carry b
+        rec_enter<S, CPL, NW, RW, C4>(Q, pf[PB], th, P >= 1 && P <= g.nz, L);
+        sfor<RW>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            sfor<CPL>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const float ri = Q.Iq[S][t][q], rv = Q.Rq[S][t][q];
+                if constexpr (((q + t + S) & 1) == 1) Q.X[S][t][q] = (ri != 0.0f) ? rv * ri : rv;
+                else Q.X[S][t][q] = (ri != 0.0f) ? 0.0f : rv;
+            });
+        });
+        rec_issue<CPL, NW, RW, SV, C4>(pf[PB], b, rec, th, g, P + 2);
+        sfor<CPL>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
+            if constexpr (((q + RW - 1 + S) & 1) == 1) L.r.ex[EW][0][w][1][q / 2][lane] = Q.X[S][RW - 1][q];
+        });
+        // S2: plane P-1, even cells
+        if (P - 1 >= th.m0 - 1) rec_colour<0, true, CPL, NW, RW, S1, S, S2>(Q, L, ER, EW, 0, w, lane);
+        // S3: plane R = P-2: residual (odd: sum c x0_even; even: 0; isolated: b)
+        // and restriction over the plane pair (R-1, R), R even
+        const int R = P - 2;
+        if (R >= th.m0 && R <= th.last) {
+            if (R & 1) {
+#pragma unroll
+                for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+                    for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
+            }
+            sfor<RW>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                constexpr bool q0 = ((t + S2) & 1) == 1;
+                const float edge = q0 ? lane_from_left(Q.X[S2][t][CPL - 1]) : lane_from_right(Q.X[S2][t][0]);
+                const float cedge = q0 ? lane_from_left(Q.Cx[S2][t][CPL - 1]) : 0.0f;
+                sfor<CPL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const bool iso = Q.Iq[S2][t][q] == 0.0f;
+                    if constexpr (((q + t + S2) & 1) == 1) {
+                        const float sum = rec_nsum<CPL, NW, RW, t, q, S2, S1, S3>(Q, edge, cedge, L, ER, 1, w, lane);
+                        acc[t / 2][q / 2] += (iso ? 0.0f : sum) + (iso ? Q.X[S2][t][q] : 0.0f);
+                    } else {
+                        acc[t / 2][q / 2] += iso ? Q.X[S2][t][q] : 0.0f;
+                    }
+                });
+            });
+            if (!(R & 1)) {
+                const int kcc = R >> 1;
+#pragma unroll
+                for (int a = 0; a < RW / 2; a++) {
+                    if (!th.tile_row[2 * a]) continue;
+                    const int jc = (th.jrow[2 * a] + 1) >> 1;
+#pragma unroll
+                    for (int c = 0; c < CPL / 2; c++) {
+                        const int ic = (th.i0 + 2 * c + 1) >> 1;
+                        if (ic <= cg.nx && jc <= cg.ny && kcc <= cg.nz)
+                            bc[ic + jc * (int)cg.sx + kcc * (int)cg.sy] = (SV)acc[a][c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+    for (int P = th.m0 - 2; P <= th.last + 2; P += 4) {
+        step(IC<3>{}, P);
+        if (P + 1 > th.last + 2) break;
+        step(IC<0>{}, P + 1);
+        if (P + 2 > th.last + 2) break;
+        step(IC<1>{}, P + 2);
+        if (P + 3 > th.last + 2) break;
+        step(IC<2>{}, P + 3);
+    }
+}
+
+template <int CPL, int NW, int RW, typename SV, typename C4>
+__global__ __launch_bounds__(NW * 64) void k_rows_downc(const SV* __restrict__ b, const C4* __restrict__ rec, Geo g,
+                                                        RowGeo rg, SV* __restrict__ bc, Geo cg, const int* skip) {
+    d_rows_downc<CPL, NW, RW, SV, C4>(b, rec, g, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x));
+}
+
+// b (level l), coarse x -> x (level l): prolongation and reverse sweep
+template <int CPL, int NW, int RW, typename SV, typename C4>
+__device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* __restrict__ rec, const Geo& g,
+                                           const RowGeo& rg, const SV* __restrict__ xc, const Geo& cg,
+                                           SV* __restrict__ xout, const int* skip, int tile) {
+    static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
+    if (skip && *skip) return;
+    __shared__ RecLDS<CPL, NW, RW> L;
+    if (tile >= rg.ntiles) return;
+    const int ty = tile % rg.nty, kc = tile / rg.nty;
+    RecThread<CPL, NW, RW> th;
+    th.init_rec(g, rg, ty, kc);
+    const int lane = th.lane, w = th.w;
+    for (int e = threadIdx.x; e < 2 * 2 * NW * 2 * (CPL / 2) * 64; e += NW * 64) (&L.r.ex[0][0][0][0][0][0])[e] = 0.0f;
+    for (int e = threadIdx.x; e < 4 * NW * CPL * 64; e += NW * 64) (&L.cy[0][0][0][0])[e] = 0.0f;
+    __syncthreads();
+
+    RecQ<CPL, RW> Q;
+    RecPf<CPL, RW, C4> pf[2];
+    float pfx[2][RW / 2][CPL / 2];
+    bool pcin[2];
+    int coff[RW / 2][CPL / 2];
+    bool cok[RW / 2][CPL / 2];
+#pragma unroll
+    for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+        for (int c = 0; c < CPL / 2; c++) {
+            const int ic = (th.i0 + 2 * c + 1) >> 1, jc = (th.jrow[2 * a] + 1) >> 1;
+            cok[a][c] = th.okx[2 * c] && th.jrow[2 * a] >= 1 && th.jrow[2 * a] <= g.ny && ic <= cg.nx && jc >= 1 && jc <= cg.ny;
+            coff[a][c] = min(ic, cg.nx + 1) + min(max(jc, 0), cg.ny + 1) * (int)cg.sx;
+        }
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                Q.X[s4][t][q] = 0.0f; Q.Iq[s4][t][q] = 0.0f; Q.Rq[s4][t][q] = 0.0f;
+                Q.Cx[s4][t][q] = 0.0f; Q.Cy[s4][t][q] = 0.0f; Q.Cz[s4][t][q] = 0.0f;
+            }
+    }
+    auto issue = [&](int slot, int P) {
+        rec_issue<CPL, NW, RW, SV, C4>(pf[slot], b, rec, th, g, P);
+        const int kcc = (P + 1) >> 1;
+        pcin[slot] = P >= 1 && P <= g.nz && kcc >= 1 && kcc <= cg.nz;
+        const int cpo = min(max(kcc, 0), cg.nz + 1) * (int)cg.sy;
+#pragma unroll
+        for (int a = 0; a < RW / 2; a++)
+#pragma unroll
+            for (int c = 0; c < CPL / 2; c++) pfx[slot][a][c] = (float)xc[coff[a][c] + cpo];
+    };
+    issue(1, th.m0 - 2);
+    issue(0, th.m0 - 1);
+
+    auto step = [&](auto Sc, int P) {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;
+        constexpr int PB = S & 1, EW = S & 1, ER = EW ^ 1;
+        // S1: plane P: x' = x0 + 2 x_coarse on active cells
+        rec_enter<S, CPL, NW, RW, C4>(Q, pf[PB], th, P >= 1 && P <= g.nz, L);
+        sfor<RW>([&](auto tc) {
+            constexpr int t = decltype(tc)::value;
+            sfor<CPL>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const bool act = th.rowok[t] && th.okx[q] && P >= 1 && P <= g.nz && pf[PB].rec[t][q].w >= 0.0f;
+                const float w2 = (act && pcin[PB] && cok[t / 2][q / 2]) ? 2.0f * pfx[PB][t / 2][q / 2] : 0.0f;
+                if constexpr (((q + t + S) & 1) == 1) Q.X[S][t][q] = Q.Rq[S][t][q] * Q.Iq[S][t][q] + w2;
+                else Q.X[S][t][q] = w2;
+            });
+        });
+        issue(PB, P + 2);
+        sfor<CPL>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
+            if constexpr (((q + RW - 1 + S) & 1) == 1) L.r.ex[EW][0][w][1][q / 2][lane] = Q.X[S][RW - 1][q];
+        });
+        // S2: plane P-1, even cells: x = (b + sum c x'_odd) / d
+        if (P - 1 >= th.m0 - 1) rec_colour<0, true, CPL, NW, RW, S1, S, S2>(Q, L, ER, EW, 0, w, lane);
+        // S3: plane R = P-2, odd cells: x = (b + sum c x_even) / d; output
+        const int R = P - 2;
+        if (R >= th.m0 && R <= th.last) {
+            rec_colour<1, false, CPL, NW, RW, S2, S1, S3>(Q, L, ER, EW, 1, w, lane);
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                if (!th.tile_row[t]) continue;
+#pragma unroll
+                for (int q = 0; q < CPL; q++)
+                    if (th.okx[q]) xout[th.roff[t] + q + R * (int)g.sy] = (SV)Q.X[S2][t][q];
+            }
+        }
+        __syncthreads();
+    };
+    for (int P = th.m0 - 2; P <= th.last + 2; P += 4) {
+        step(IC<3>{}, P);
+        if (P + 1 > th.last + 2) break;
+        step(IC<0>{}, P + 1);
+        if (P + 2 > th.last + 2) break;
+        step(IC<1>{}, P + 2);
+        if (P + 3 > th.last + 2) break;
+        step(IC<2>{}, P + 3);
+    }
+}
+
+template <int CPL, int NW, int RW, typename SV, typename C4>
+__global__ __launch_bounds__(NW * 64) void k_rows_upc(const SV* __restrict__ b, const C4* __restrict__ rec, Geo g,
+                                                      RowGeo rg, const SV* __restrict__ xc, Geo cg,
+                                                      SV* __restrict__ xout, const int* skip) {
+    d_rows_upc<CPL, NW, RW, SV, C4>(b, rec, g, rg, xc, cg, xout, skip, row_tile_of_block(rg, blockIdx.x));
 }
 
 }  // namespace uaamg

@@ -43,20 +43,23 @@ void Engine<D, T, S>::allocate() {
     lv_[0].g = g0_;
     lv_[0].lc = lc0_;
     solid0_ = dalloc<uint8_t>(g0_.N);
-    nbr0_ = dalloc<uint8_t>(N4_);
+    // kRowSlack: the row kernels load CPL consecutive cells per lane from a
+    // clamped row start, up to CPL - 1 cells past the last ghost cell
+    nbr0_ = dalloc<uint8_t>(N4_ + kRowSlack);
     count_ = dalloc<double>(1);
     ctrl_ = dalloc<Ctrl>(1);
     HIPCHK(hipHostMalloc((void**)&ctrl_host_, sizeof(Ctrl), hipHostMallocDefault));
     r_ = dalloc_margin<T>(N4_); z_ = dalloc_margin<T>(N4_); p_ = dalloc_margin<T>(N4_); Ap_ = dalloc_margin<T>(N4_);
     if (!x_is_p()) x_ = dalloc_margin<T>(N4_);
+    if (pmv_on_) { p2_ = dalloc_margin<T>(N4_); r2_ = dalloc_margin<T>(N4_); }
     long maxb = std::max<long>(lc0_.nblocks(), 2048);
     for (int l = 1; l < nl; l++) {
         Level& L = lv_[l];
         L.g = shape_[l];
         L.lc = cells_launch(L.g, D);
-        L.b = dalloc<S>(L.g.N); L.x = dalloc<S>(L.g.N);
+        L.b = dalloc<S>(L.g.N + kRowSlack); L.x = dalloc<S>(L.g.N + kRowSlack);
         L.mask = dalloc<uint8_t>(L.g.N);
-        L.coef = dalloc<C4>(L.g.N);
+        L.coef = dalloc<C4>(L.g.N + kRowSlack);
         L.ex = dalloc<S>(L.g.N);
         maxb = std::max(maxb, L.lc.nblocks());
     }
@@ -465,10 +468,13 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     const Geo& g0 = shape[0];
     const size_t N0 = (size_t)g0.N, N4 = (N0 + 3) / 4 * 4;
     const size_t margin = (size_t)((g0.sx + 8 + 63) / 64) * 64;
-    size_t bytes = N0 + N4 + 8 + sizeof(uaamg::Ctrl) + 5 * (N4 + 2 * margin) * sv;
+    // level-0 vectors r, z, p, Ap, x and the second p and r buffers of the
+    // fused CG kernels (3D FP32 forms with rows of at most 256 cells)
+    const bool pmv = g->dim == 3 && o->precision != UAAMG_FP64 && g0.nx <= 256;
+    size_t bytes = N0 + N4 + kRowSlack + 8 + sizeof(uaamg::Ctrl) + (pmv ? 7 : 5) * (N4 + 2 * margin) * sv;
     long maxb = std::max<long>(cells_launch(g0, g->dim).nblocks(), 2048);
     for (size_t l = 1; l < shape.size(); l++) {
-        bytes += (size_t)shape[l].N * (7 * sl + 1);
+        bytes += (size_t)shape[l].N * (7 * sl + 1) + (size_t)kRowSlack * 6 * sl;
         maxb = std::max(maxb, cells_launch(shape[l], g->dim).nblocks());
     }
     // fused level-0 launch (at most) and the row kernels' workgroups
