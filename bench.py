@@ -32,17 +32,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def per_cell_bytes(cls, sv, sl, dim):
+def per_cell_bytes(cls, sv, sl, dim, mode):
     """Algorithmic HBM bytes per interior cell for one launch of a level-0 kernel
-    class (each array read or written once; nbr = 1-byte neighbour mask)."""
+    class (each array read or written once; nbr = 1-byte neighbour mask).
+    mode "plain": operator, CG update, down, up, p update as five kernels;
+    "pmv": the p update fused into the operator pass; "split": in addition the
+    x update rides on the next operator pass and the r update on the down
+    kernel (no separate update kernel)."""
     rho = 1.0 / 8 if dim == 3 else 1.0 / 4
-    return {
+    table = {
         "matvec_dot": 2 * sv + 1,          # read p, nbr; write Ap
         "update": 6 * sv + 1,              # read x, p, r, Ap, nbr; write x, r
         "down0": sv + 1 + rho * sl,        # read r, nbr; write coarse b
         "up0": 2 * sv + 1 + rho * sl,      # read r, nbr, coarse x; write z
         "pupdate": 3 * sv + 1,             # read z, p, nbr; write p
-    }.get(cls)
+    }
+    if mode in ("pmv", "split"):
+        table["matvec_dot"] = 4 * sv + 1   # read z, p_old, nbr; write p_new, Ap
+    if mode == "split":
+        table["matvec_dot"] += 2 * sv      # + read / write x
+        table["down0"] = 3 * sv + 1 + rho * sl   # read r_old, Ap, nbr; write r_new, coarse b
+    return table.get(cls)
 
 
 def cpu_baseline(pb, k_iters, budget_s=30.0):
@@ -507,13 +517,18 @@ def main():
             log(f"[bench] {KNAMES[c]:>10s}: {ms_k[c] / n_k[c] * 1e3:8.1f} us/launch x {int(n_k[c] / args.steps)} per solve")
     sv = 8 if prec == U.FP64 else 4
     sl = {U.FP64: 8, U.FP32: 4}.get(prec, 2)
+    # which level-0 kernel set ran: no p-update launches = fused into the
+    # operator pass; no update launches either = split update
+    mode = "plain"
+    if n_k[U.capi.K_PUPDATE] == 0:
+        mode = "split" if n_k[U.capi.K_UPDATE] == 0 else "pmv"
     cand = [c for c in range(5) if n_k[c] > 0]
     dom = max(cand, key=lambda c: ms_k[c])
     dname = KNAMES[dom]
     avg_s = ms_k[dom] / n_k[dom] / 1e3
-    bytes_launch = per_cell_bytes(dname, sv, sl, pb.dim) * pb.cells
+    bytes_launch = per_cell_bytes(dname, sv, sl, pb.dim, mode) * pb.cells
     achieved = bytes_launch / avg_s / 1e9
-    traffic = load_traffic(f"{pb.name}:{args.precision}:{dname}")
+    traffic = load_traffic(f"{pb.name}:{args.precision}:{mode}:{dname}")
 
     k_med = int(np.median(iters))
     out = {
@@ -542,7 +557,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dname,
+            "kernel": {"matvec_dot": "p update + operator (k_rows_pmv)" if mode != "plain" else "operator (k_matvec_dot)",
+                       "update": "CG update (k_update)", "down0": "level-0 down (k_rows_down0" + ("r)" if mode == "split" else ")"),
+                       "up0": "level-0 up (k_rows_up0)", "pupdate": "p update (k_pupdate)"}[dname],
+            "kernel_mode": mode,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

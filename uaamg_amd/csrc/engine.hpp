@@ -292,9 +292,11 @@ public:
         unsigned ta = (unsigned)((g.nx + TX - 1) / TX), tb = (D == 3) ? (unsigned)((g.ny + TY - 1) / TY) : 1u;
         int nm = (D == 3) ? g.nz : g.ny;
         long tiles = (long)ta * tb;
+        // small levels are latency-bound: short chunks (down to 2 planes) give
+        // more workgroups and fewer sequential plane steps
         static const int kc_min = [] {
             const char* e = std::getenv("UAAMG_MARCH_KCMIN");
-            return e ? std::max(2, std::atoi(e) / 2 * 2) : 8;
+            return e ? std::max(2, std::atoi(e) / 2 * 2) : 2;
         }();
         static const long tmul = [] {
             const char* e = std::getenv("UAAMG_MARCH_TARGET_X");
