@@ -149,6 +149,23 @@ int uaamg_level_data(uaamg_ctx* ctx, int level, uint8_t* solid, double* diag,
 int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, double tol,
                  int* iters, int precision);
 
+/* Pressure projection around the solve: replaces PressureProjection::project
+ * (src/pressure/pressure.cpp:5-32) and PressureProjection3D::project
+ * (src/pressure/pressure_3d.cpp:5-39) with the pcg_uaamg solver.  On the grid
+ * set by uaamg_set_grid: rhs = div(u)/dt on non-solid interior cells
+ * (fvc::divergence, operators.h:18-21, operators_3d.h:17-21), p = 0, the
+ * solve (as uaamg_solve_device, cold start), then u -= dt (p+ - p-)/dx on
+ * interior faces between two non-solid cells (and v, w).  u, v, w are the
+ * reference's MAC arrays (mesh.h:27-35, mesh_3d.h:29-37): u (nx+1)(ny+2)[(nz+2)],
+ * v (nx+2)(ny+1)[(nz+2)], w (nx+2)(ny+2)(nz+1) (ignored in 2D); p is the
+ * padded cell array.  The correction is applied whatever the solve status, as
+ * the reference does.  _device: device pointers; otherwise host arrays
+ * (copied in and out). */
+int uaamg_project_device(uaamg_ctx* ctx, double* u, double* v, double* w, double* p, double dt,
+                         int max_iter, double tol, uaamg_stats* st);
+int uaamg_project(uaamg_ctx* ctx, double* u, double* v, double* w, double* p, double dt,
+                  int max_iter, double tol, uaamg_stats* st);
+
 /* Device memory (bytes) a context would hold for grid g under opts o. */
 size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o);
 

@@ -12,6 +12,8 @@
 #include "solver/pcg_3d.h"
 #include "solver/preconditioner/uaamg_preconditioner.h"
 #include "solver/preconditioner/uaamg_preconditioner_3d.h"
+#include "pressure/pressure.h"
+#include "pressure/pressure_3d.h"
 #include <memory>
 #include <vector>
 
@@ -79,3 +81,35 @@ extern "C" int ref_uaamg_apply(int dim, int nx, int ny, int nz, double lx, doubl
 
 // dx exactly as the reference's Grid computes it, for building matching inputs.
 extern "C" double ref_spacing(double l, int n) { Grid g(n, 2, l, 1.0); return g.dx; }
+
+// PressureProjection[3D]::project (src/pressure/pressure.cpp:5-32,
+// pressure_3d.cpp:5-39) with the pcg_uaamg solver on MAC velocities in the
+// Grid's own layout; u, v, w, p are copied in and out.
+extern "C" int ref_project(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                           const unsigned char* solid, double* u, double* v, double* w, double* p,
+                           double dt, int max_iter, double tol) {
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        for (size_t c = 0; c < g.p.size(); c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+        for (size_t c = 0; c < g.u.size(); c++) g.u[c] = u[c];
+        for (size_t c = 0; c < g.v.size(); c++) g.v[c] = v[c];
+        PCG s(std::make_unique<UAAMGPreconditioner>());
+        PressureProjection::project(g, dt, s, max_iter, tol);
+        for (size_t c = 0; c < g.p.size(); c++) p[c] = g.p[c];
+        for (size_t c = 0; c < g.u.size(); c++) u[c] = g.u[c];
+        for (size_t c = 0; c < g.v.size(); c++) v[c] = g.v[c];
+        return 0;
+    }
+    Grid3D g(nx, ny, nz, lx, ly, lz);
+    for (size_t c = 0; c < g.p.size(); c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+    for (size_t c = 0; c < g.u.size(); c++) g.u[c] = u[c];
+    for (size_t c = 0; c < g.v.size(); c++) g.v[c] = v[c];
+    for (size_t c = 0; c < g.w.size(); c++) g.w[c] = w[c];
+    PCG3D s(std::make_unique<UAAMGPreconditioner3D>());
+    PressureProjection3D::project(g, dt, s, max_iter, tol);
+    for (size_t c = 0; c < g.p.size(); c++) p[c] = g.p[c];
+    for (size_t c = 0; c < g.u.size(); c++) u[c] = g.u[c];
+    for (size_t c = 0; c < g.v.size(); c++) v[c] = g.v[c];
+    for (size_t c = 0; c < g.w.size(); c++) w[c] = g.w[c];
+    return 0;
+}

@@ -655,3 +655,78 @@ int orc_amgpcg(int N, const double* bin, double* xout, int max_iter, double tol,
     free(A); free(x.v); free(r.v); free(z.v); free(p.v); free(Ap.v);
     return 0;
 }
+
+/* ---------------------------------------------------------------- projection
+ * PressureProjection::project (src/pressure/pressure.cpp:5-32) and
+ * PressureProjection3D::project (src/pressure/pressure_3d.cpp:5-39):
+ *   1. rhs = div(u)/dt on non-solid interior cells (fvc::divergence,
+ *      include/ops/operators.h:18-21, operators_3d.h:17-21), 0 elsewhere;
+ *   2. p = 0 everywhere, then the PCG(UAAMG) solve (orc_pcg);
+ *   3. on interior faces between two non-solid cells, u -= dt*(p+ - p-)/dx
+ *      (and v, w), in the reference's loop ranges.
+ * MAC layouts of mesh.h:27-35 / mesh_3d.h:29-37. */
+static size_t mu(const orc_grid* g, int i, int j, int k) {
+    return (size_t)i + (size_t)j * (g->nx + 1) + (size_t)k * (g->nx + 1) * (g->ny + 2);
+}
+static size_t mv(const orc_grid* g, int i, int j, int k) {
+    return (size_t)i + (size_t)j * (g->nx + 2) + (size_t)k * (g->nx + 2) * (g->ny + 1);
+}
+static size_t mw(const orc_grid* g, int i, int j, int k) {
+    return (size_t)i + (size_t)j * (g->nx + 2) + (size_t)k * (g->nx + 2) * (g->ny + 2);
+}
+static size_t mp(const orc_grid* g, int i, int j, int k) {
+    return (size_t)i + (size_t)j * (g->nx + 2) + (size_t)k * (g->nx + 2) * (g->ny + 2);
+}
+
+void orc_project_rhs(const orc_grid* g, const double* u, const double* v, const double* w, double dt, double* rhs) {
+    const int k0 = g->dim == 3 ? 1 : 0, k1 = g->dim == 3 ? g->nz : 0;
+    memset(rhs, 0, padded_size(g) * sizeof(double));
+    for (int k = k0; k <= k1; k++)
+        for (int j = 1; j <= g->ny; j++)
+            for (int i = 1; i <= g->nx; i++) {
+                if (g->solid[mp(g, i, j, k)]) continue;
+                double div = (u[mu(g, i, j, k)] - u[mu(g, i - 1, j, k)]) / g->dx
+                           + (v[mv(g, i, j, k)] - v[mv(g, i, j - 1, k)]) / g->dy;
+                if (g->dim == 3) div = div + (w[mw(g, i, j, k)] - w[mw(g, i, j, k - 1)]) / g->dz;
+                rhs[mp(g, i, j, k)] = div / dt;
+            }
+}
+
+void orc_project_correct(const orc_grid* g, double* u, double* v, double* w, const double* p, double dt) {
+    const int k0 = g->dim == 3 ? 1 : 0, k1 = g->dim == 3 ? g->nz : 0;
+    const unsigned char* s = g->solid;
+    for (int k = k0; k <= k1; k++)
+        for (int j = 1; j <= g->ny; j++)
+            for (int i = 1; i < g->nx; i++) {
+                if (s[mp(g, i, j, k)] || s[mp(g, i + 1, j, k)]) continue;
+                u[mu(g, i, j, k)] -= dt * (p[mp(g, i + 1, j, k)] - p[mp(g, i, j, k)]) / g->dx;
+            }
+    for (int k = k0; k <= k1; k++)
+        for (int j = 1; j < g->ny; j++)
+            for (int i = 1; i <= g->nx; i++) {
+                if (s[mp(g, i, j, k)] || s[mp(g, i, j + 1, k)]) continue;
+                v[mv(g, i, j, k)] -= dt * (p[mp(g, i, j + 1, k)] - p[mp(g, i, j, k)]) / g->dy;
+            }
+    if (g->dim == 3)
+        for (int k = 1; k < g->nz; k++)
+            for (int j = 1; j <= g->ny; j++)
+                for (int i = 1; i <= g->nx; i++) {
+                    if (s[mp(g, i, j, k)] || s[mp(g, i, j, k + 1)]) continue;
+                    w[mw(g, i, j, k)] -= dt * (p[mp(g, i, j, k + 1)] - p[mp(g, i, j, k)]) / g->dz;
+                }
+}
+
+int orc_project(const orc_grid* g, const orc_opts* o, double* u, double* v, double* w, double* p, double dt,
+                int max_iter, double tol, orc_result* res) {
+    if (check_grid(g)) return -1;
+    size_t N = padded_size(g);
+    double* rhs = malloc(N * sizeof(double));
+    if (!rhs) return -2;
+    orc_project_rhs(g, u, v, w, dt, rhs);
+    memset(p, 0, N * sizeof(double));
+    int rc = orc_pcg(g, o, rhs, p, max_iter, tol, res);
+    free(rhs);
+    if (rc) return rc;
+    orc_project_correct(g, u, v, w, p, dt);
+    return 0;
+}

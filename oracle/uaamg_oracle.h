@@ -77,6 +77,16 @@ int orc_matvec(const orc_grid* g, const double* v, double* Av);
  * row-major v[i*(N+2)+j]. b, x: (N+2)^2 arrays. */
 int orc_amgpcg(int N, const double* b, double* x, int max_iter, double tol, int* iters);
 
+/* Pressure projection (pressure.cpp:5-32, pressure_3d.cpp:5-39) on MAC
+ * velocities: u (nx+1)(ny+2)[(nz+2)], v (nx+2)(ny+1)[(nz+2)],
+ * w (nx+2)(ny+2)(nz+1) (mesh.h:27-35, mesh_3d.h:29-37; w unused in 2D),
+ * p padded.  rhs = div(u)/dt, p = 0, orc_pcg, then u -= dt grad p on faces
+ * between two non-solid cells.  The two halves are exported for tests. */
+int orc_project(const orc_grid* g, const orc_opts* o, double* u, double* v, double* w, double* p, double dt,
+                int max_iter, double tol, orc_result* res);
+void orc_project_rhs(const orc_grid* g, const double* u, const double* v, const double* w, double dt, double* rhs);
+void orc_project_correct(const orc_grid* g, double* u, double* v, double* w, const double* p, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,3 +106,44 @@ def parity_cases():
 def neumann_cases():
     """parity_cases() with all walls Neumann (the reference's own setting)."""
     return [(n, pb) for n, pb in parity_cases() if all(w == ob.NEUMANN for w in pb.wall)]
+
+
+def mac_fields(pb, rng, closed=True):
+    """Random MAC velocities u, v[, w] in the reference layouts (mesh.h:27-35,
+    mesh_3d.h:29-37).  closed: zero normal velocity on the box walls and on
+    faces touching a solid cell, so every fluid region has zero net flux and
+    the projected field can be divergence-free."""
+    nx, ny = pb.n[0], pb.n[1]
+    nz = pb.n[2] if pb.dim == 3 else 1
+    kk = nz + 2 if pb.dim == 3 else 1
+    s = pb.solid.reshape(pb.shape[::-1]).astype(bool)          # [k][j][i] or [j][i]
+    if pb.dim == 2:
+        s = s[None]
+    u = rng.standard_normal((kk, ny + 2, nx + 1))
+    v = rng.standard_normal((kk, ny + 1, nx + 2))
+    w = rng.standard_normal((nz + 1, ny + 2, nx + 2)) if pb.dim == 3 else None
+    if closed:
+        u[:, :, 0] = 0; u[:, :, nx] = 0
+        v[:, 0, :] = 0; v[:, ny, :] = 0
+        u[:, :, :] *= ~(s[:, :, :-1] | s[:, :, 1:])            # face i between cells i and i+1
+        v[:, :, :] *= ~(s[:, :-1, :] | s[:, 1:, :])
+        if w is not None:
+            w[0] = 0; w[nz] = 0
+            w[:, :, :] *= ~(s[:-1] | s[1:])
+    return u.reshape(-1), v.reshape(-1), (w.reshape(-1) if w is not None else None)
+
+
+def mac_divergence(pb, u, v, w):
+    """fvc::divergence on every interior cell (operators.h:18-21, operators_3d.h:17-21), padded array."""
+    nx, ny = pb.n[0], pb.n[1]
+    nz = pb.n[2] if pb.dim == 3 else 1
+    kk = nz + 2 if pb.dim == 3 else 1
+    U = u.reshape(kk, ny + 2, nx + 1)
+    V = v.reshape(kk, ny + 1, nx + 2)
+    d = np.zeros((kk, ny + 2, nx + 2))
+    ks = slice(1, -1) if pb.dim == 3 else slice(0, 1)
+    d[ks, 1:-1, 1:-1] = (U[ks, 1:-1, 1:] - U[ks, 1:-1, :-1]) / pb.h[0] + (V[ks, 1:, 1:-1] - V[ks, :-1, 1:-1]) / pb.h[1]
+    if pb.dim == 3:
+        W = w.reshape(nz + 1, ny + 2, nx + 2)
+        d[1:-1, 1:-1, 1:-1] += (W[1:, 1:-1, 1:-1] - W[:-1, 1:-1, 1:-1]) / pb.h[2]
+    return d.reshape(-1)

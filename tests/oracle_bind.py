@@ -47,6 +47,9 @@ def lib():
         _lib.orc_level_data.argtypes = [P, P, C.c_int, P, P, P, P, P, P]
         _lib.orc_matvec.argtypes = [P, P, P]
         _lib.orc_amgpcg.argtypes = [C.c_int, P, P, C.c_int, C.c_double, P]
+        _lib.orc_project.argtypes = [P, P, P, P, P, P, C.c_double, C.c_int, C.c_double, P]
+        _lib.orc_project_rhs.argtypes = [P, P, P, P, C.c_double, P]
+        _lib.orc_project_correct.argtypes = [P, P, P, P, P, C.c_double]
     return _lib
 
 
@@ -65,6 +68,7 @@ def ref():
         _ref.ref_amgpcg.argtypes = [I, P, P, I, D, P]
         _ref.ref_spacing.argtypes = [D, I]
         _ref.ref_spacing.restype = D
+        _ref.ref_project.argtypes = [I, I, I, I, D, D, D, P, P, P, P, P, D, I, D]
     return _ref
 
 
@@ -109,6 +113,14 @@ class Problem:
         if self.dim == 3:
             return v[1:-1, 1:-1, 1:-1]
         return v[1:-1, 1:-1]
+
+    def mac_sizes(self):
+        """Sizes of the MAC velocity arrays u, v[, w] (mesh.h:27-29, mesh_3d.h:29-31)."""
+        nx, ny = self.n[0], self.n[1]
+        if self.dim == 2:
+            return ((nx + 1) * (ny + 2), (nx + 2) * (ny + 1), 1)
+        nz = self.n[2]
+        return ((nx + 1) * (ny + 2) * (nz + 2), (nx + 2) * (ny + 1) * (nz + 2), (nx + 2) * (ny + 2) * (nz + 1))
 
     def fluid_interior_mask(self):
         m = np.zeros(self.N, bool)
@@ -213,3 +225,40 @@ def ref_amgpcg(N, b, max_iter, tol):
     it = C.c_int(0)
     ref().ref_amgpcg(N, _p(b), _p(x), max_iter, tol, C.byref(it))
     return x, it.value
+
+
+def project(pb, u, v, w, dt, max_iter, tol, **opts):
+    """Pressure projection (oracle): returns (u, v, w, p, result)."""
+    u = np.array(u, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    w = np.array(w if w is not None else np.zeros(1), dtype=np.float64)
+    p = np.zeros(pb.N)
+    g = pb.grid()
+    o = default_opts(**opts)
+    res = OrcResult()
+    rc = lib().orc_project(C.byref(g), C.byref(o), _p(u), _p(v), _p(w), _p(p), dt, max_iter, tol, C.byref(res))
+    assert rc == 0, rc
+    return u, v, w, p, res
+
+
+def project_rhs(pb, u, v, w, dt):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    w = np.ascontiguousarray(w if w is not None else np.zeros(1), dtype=np.float64)
+    rhs = np.zeros(pb.N)
+    g = pb.grid()
+    lib().orc_project_rhs(C.byref(g), _p(u), _p(v), _p(w), dt, _p(rhs))
+    return rhs
+
+
+def ref_project(pb, u, v, w, dt, max_iter, tol):
+    """The reference's PressureProjection[3D]::project with the pcg_uaamg solver."""
+    u = np.array(u, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    w = np.array(w if w is not None else np.zeros(1), dtype=np.float64)
+    p = np.zeros(pb.N)
+    nz = pb.n[2] if pb.dim == 3 else 1
+    lz = pb.L[2] if pb.dim == 3 else 1.0
+    ref().ref_project(pb.dim, pb.n[0], pb.n[1], nz, pb.L[0], pb.L[1], lz, _p(pb.solid), _p(u), _p(v), _p(w), _p(p),
+                      dt, max_iter, tol)
+    return u, v, w, p

@@ -270,3 +270,73 @@ def test_dirichlet_extension_second_order():
         exact = -np.sin(np.pi * X) * np.sin(np.pi * Y)
         errs.append(np.abs(1e-4 * p.reshape(n + 2, n + 2)[1:-1, 1:-1] - exact[1:-1, 1:-1]).max())
     assert errs[1] < errs[0] / 3.2 and errs[2] < errs[1] / 3.2
+
+
+# ---- pressure projection (pressure.cpp:5-32, pressure_3d.cpp:5-39) ----
+
+PROJ_CASES = [c for c in cases.neumann_cases() if c[0] in
+              ("2d_32x16", "2d_64x16_cylinder", "2d_33x17_odd", "3d_16_sphere", "3d_17x9x5_odd",
+               "3d_24x20x18_aniso_solids")]
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("name,pb", PROJ_CASES, ids=[c[0] for c in PROJ_CASES])
+def test_project_matches_reference(name, pb):
+    """The oracle's projection equals the reference's PressureProjection[3D]
+    ::project with the pcg_uaamg solver, bit for bit (u, v, w and p)."""
+    rng = np.random.default_rng(5)
+    u, v, w = cases.mac_fields(pb, rng, closed=False)
+    dt = 0.01
+    uo, vo, wo, po, res = ob.project(pb, u, v, w, dt, 200, 1e-10)
+    ur, vr, wr, pr = ob.ref_project(pb, u, v, w, dt, 200, 1e-10)
+    np.testing.assert_array_equal(po, pr)
+    np.testing.assert_array_equal(uo, ur)
+    np.testing.assert_array_equal(vo, vr)
+    if pb.dim == 3:
+        np.testing.assert_array_equal(wo, wr)
+
+
+@pytest.mark.parametrize("name,pb", PROJ_CASES, ids=[c[0] for c in PROJ_CASES])
+def test_project_divergence_free(name, pb):
+    """After the projection of a closed field (no flux through walls or solid
+    faces) the discrete divergence vanishes on fluid cells to the solver's
+    tolerance: this fixes the rhs sign and scaling and the faces and sign of
+    the correction (a wrong index or sign leaves O(1) divergence)."""
+    rng = np.random.default_rng(6)
+    u, v, w = cases.mac_fields(pb, rng, closed=True)
+    dt = 0.02
+    d0 = cases.mac_divergence(pb, u, v, w)
+    fl = pb.fluid_interior_mask()
+    uo, vo, wo, po, res = ob.project(pb, u, v, w, dt, 500, 1e-9 * np.abs(d0[fl]).max() / dt)
+    # converged, or the reference's silent stop at p.Ap < 1e-15 (pcg.cpp:100),
+    # which this absolute threshold reaches once r is at round-off
+    assert res.status in (0, 2), res.status
+    d1 = cases.mac_divergence(pb, uo, vo, wo)
+    assert np.abs(d1[fl]).max() <= 1e-8 * np.abs(d0[fl]).max(), (np.abs(d1[fl]).max(), np.abs(d0[fl]).max())
+    # faces touching a solid cell and the wall faces are left as they were
+    nx = pb.n[0]
+    U0 = u.reshape(-1, nx + 1)
+    U1 = uo.reshape(-1, nx + 1)
+    np.testing.assert_array_equal(U1[:, 0], U0[:, 0])
+    np.testing.assert_array_equal(U1[:, nx], U0[:, nx])
+
+
+def test_project_rhs_linear_field():
+    """u = a x, v = b y, w = c z (closed form): div = a + b + c on every
+    fluid cell, so rhs = (a + b + c)/dt there and 0 on solid cells."""
+    pb = ob.Problem((12, 10, 8), (1.2, 0.5, 0.4))
+    s = np.zeros(pb.shape[::-1], np.uint8)
+    s[3:5, 4:6, 5:8] = 1
+    pb.solid = s.reshape(-1)
+    nx, ny, nz = pb.n
+    a, b, c, dt = 2.0, -3.0, 0.5, 0.25
+    xf = np.arange(nx + 1) * pb.h[0]
+    yf = np.arange(ny + 1) * pb.h[1]
+    zf = np.arange(nz + 1) * pb.h[2]
+    u = np.broadcast_to(a * xf, (nz + 2, ny + 2, nx + 1)).reshape(-1)
+    v = np.broadcast_to((b * yf)[None, :, None], (nz + 2, ny + 1, nx + 2)).reshape(-1)
+    w = np.broadcast_to((c * zf)[:, None, None], (nz + 1, ny + 2, nx + 2)).reshape(-1)
+    rhs = ob.project_rhs(pb, u, v, w, dt)
+    fl = pb.fluid_interior_mask()
+    np.testing.assert_allclose(rhs[fl], (a + b + c) / dt, rtol=1e-12)
+    assert np.all(rhs[~fl] == 0)

@@ -42,7 +42,7 @@ class Stats(C.Structure):
 EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",
            "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_solve", "uaamg_solve_device",
            "uaamg_apply_precond", "uaamg_matvec", "uaamg_hierarchy", "uaamg_level_data",
-           "uaamg_amgpcg", "uaamg_bytes_required",
+           "uaamg_amgpcg", "uaamg_bytes_required", "uaamg_project", "uaamg_project_device",
            "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
            "uaamg_batch_add_device", "uaamg_batch_clear", "uaamg_batch_size", "uaamg_batch_bytes",
            "uaamg_batch_solve",
@@ -80,6 +80,8 @@ class Lib:
         L.uaamg_amgpcg.argtypes = [I, P, P, I, D, P, I]
         L.uaamg_bytes_required.argtypes = [P, P]
         L.uaamg_bytes_required.restype = C.c_size_t
+        L.uaamg_project.argtypes = [P, P, P, P, P, D, I, D, P]
+        L.uaamg_project_device.argtypes = [P, P, P, P, P, D, I, D, P]
         L.uaamg_batch_create.argtypes = [C.POINTER(C.c_void_p), P]
         L.uaamg_batch_destroy.argtypes = [P]
         L.uaamg_batch_destroy.restype = None
@@ -206,6 +208,28 @@ class Context:
                                            int(max_iter), float(tol), C.byref(st))
         if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
             self._err("solve_device", rc)
+        return st
+
+    def project(self, u, v, w, dt, max_iter=-1, tol=-1.0):
+        """Pressure projection on host MAC arrays (copies); returns (u, v, w, p, Stats)."""
+        u = np.array(u, dtype=np.float64)
+        v = np.array(v, dtype=np.float64)
+        w = np.array(w, dtype=np.float64) if w is not None else None
+        p = np.zeros(self.N)
+        st = Stats()
+        rc = self.lib.L.uaamg_project(self.h, _p(u), _p(v), _p(w) if w is not None else None, _p(p), float(dt),
+                                      int(max_iter), float(tol), C.byref(st))
+        if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
+            self._err("project", rc)
+        return u, v, w, p, st
+
+    def project_device(self, u_ptr, v_ptr, w_ptr, p_ptr, dt, max_iter=-1, tol=-1.0):
+        st = Stats()
+        rc = self.lib.L.uaamg_project_device(self.h, C.c_void_p(u_ptr), C.c_void_p(v_ptr),
+                                             C.c_void_p(w_ptr) if w_ptr else None, C.c_void_p(p_ptr), float(dt),
+                                             int(max_iter), float(tol), C.byref(st))
+        if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
+            self._err("project_device", rc)
         return st
 
     def apply(self, r):

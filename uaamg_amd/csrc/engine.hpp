@@ -117,6 +117,7 @@ public:
     virtual size_t bytes() const = 0;
     virtual void set_opts(const uaamg_opts& o) = 0;
     virtual void set_stream(hipStream_t s) = 0;
+    virtual const uint8_t* solid_dev() const = 0;   // the level-0 solid mask (padded, device)
 };
 
 // Hierarchy shape, shared by engines and the byte estimator (buildHierarchy,
@@ -199,6 +200,7 @@ public:
         destroy_graph();
     }
     void set_stream(hipStream_t s) override { stream_ = s; timer_.s = s; }
+    const uint8_t* solid_dev() const override { return solid0_; }
     int nlevels() const override { return (int)shape_.size(); }
     Geo geo(int l) const override { return shape_[l]; }
     size_t bytes() const override { return bytes_; }

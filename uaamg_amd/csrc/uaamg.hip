@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <deque>
 #include <unordered_map>
 #include <cstdio>
@@ -9,6 +10,7 @@
 #include <memory>
 #include <string>
 #include "../../include/uaamg.h"
+#include "kernels_proj.hpp"
 #include "batch.hpp"
 #include "engine.hpp"
 #include "amgpcg.hpp"
@@ -366,6 +368,81 @@ extern "C" int uaamg_solve_device(uaamg_ctx* c, const double* rhs_dev, double* p
     if (st) std::memset(st, 0, sizeof(*st));
     int rc = UAAMG_OK;
     CABI_TRY(c, { rc = c->eng->solve(rhs_dev, p_dev, warm_start ? 1 : 0, max_iter, tol, st); })
+    return rc;
+}
+
+// ---------------------------------------------------------------- projection
+static size_t mac_n(const uaamg_ctx* c, int comp) {
+    const long nx = c->n[0], ny = c->n[1], nz = c->dim == 3 ? c->n[2] : 0;
+    if (c->dim == 2) return comp == 0 ? (nx + 1) * (ny + 2) : comp == 1 ? (nx + 2) * (ny + 1) : 0;
+    return comp == 0 ? (nx + 1) * (ny + 2) * (nz + 2) : comp == 1 ? (nx + 2) * (ny + 1) * (nz + 2) : (nx + 2) * (ny + 2) * (nz + 1);
+}
+
+static int project_impl(uaamg_ctx* c, double* u, double* v, double* w, double* p, double dt, int max_iter,
+                        double tol, uaamg_stats* st) {
+    const Geo g = c->eng->geo(0);
+    const MacGeo m = make_mac(g);
+    const uint8_t* solid = c->eng->solid_dev();
+    const unsigned blocks = (unsigned)std::min<long>((g.N + 255) / 256, 8192);
+    const double dz = c->dim == 3 ? c->h[2] : 1.0;
+    ensure_stage(c);
+    if (c->dim == 3)
+        k_proj_rhs<3><<<blocks, 256, 0, c->stream>>>(u, v, w, solid, c->rhs_stage, g, m, c->h[0], c->h[1], dz, dt);
+    else
+        k_proj_rhs<2><<<blocks, 256, 0, c->stream>>>(u, v, w, solid, c->rhs_stage, g, m, c->h[0], c->h[1], dz, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(p, 0, g.N * sizeof(double), c->stream));   // std::fill(g.p, 0), pressure.cpp:17
+    int rc = c->eng->solve(c->rhs_stage, p, 0, max_iter, tol, st);
+    if (c->dim == 3)
+        k_proj_correct<3><<<blocks, 256, 0, c->stream>>>(u, v, w, p, solid, g, m, c->h[0], c->h[1], dz, dt);
+    else
+        k_proj_correct<2><<<blocks, 256, 0, c->stream>>>(u, v, w, p, solid, g, m, c->h[0], c->h[1], dz, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+extern "C" int uaamg_project_device(uaamg_ctx* c, double* u, double* v, double* w, double* p, double dt,
+                                    int max_iter, double tol, uaamg_stats* st) {
+    if (!c) return UAAMG_ERR_ARG;
+    if (!c->eng) return fail(c, UAAMG_ERR_ARG, "no grid set");
+    if (!u || !v || !p || (c->dim == 3 && !w)) return fail(c, UAAMG_ERR_ARG, "null array");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, UAAMG_ERR_ARG, "dt must be positive");
+    if (max_iter < 0) max_iter = c->o.max_iter;
+    if (tol < 0) tol = c->o.tol;
+    if (st) std::memset(st, 0, sizeof(*st));
+    int rc = UAAMG_OK;
+    CABI_TRY(c, { rc = project_impl(c, u, v, w, p, dt, max_iter, tol, st); })
+    return rc;
+}
+
+extern "C" int uaamg_project(uaamg_ctx* c, double* u, double* v, double* w, double* p, double dt,
+                             int max_iter, double tol, uaamg_stats* st) {
+    if (!c) return UAAMG_ERR_ARG;
+    if (!c->eng) return fail(c, UAAMG_ERR_ARG, "no grid set");
+    if (!u || !v || !p || (c->dim == 3 && !w)) return fail(c, UAAMG_ERR_ARG, "null array");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, UAAMG_ERR_ARG, "dt must be positive");
+    if (max_iter < 0) max_iter = c->o.max_iter;
+    if (tol < 0) tol = c->o.tol;
+    if (st) std::memset(st, 0, sizeof(*st));
+    int rc = UAAMG_OK;
+    double* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    const size_t nb[4] = {mac_n(c, 0), mac_n(c, 1), mac_n(c, 2), (size_t)padded_n(c)};
+    double* host[4] = {u, v, c->dim == 3 ? w : nullptr, p};
+    auto release = [&]() { for (auto& b : buf) if (b) { (void)hipFree(b); b = nullptr; } };
+    try {
+        for (int q = 0; q < 4; q++) {
+            if (!host[q]) continue;
+            HIPCHK(hipMalloc(&buf[q], nb[q] * sizeof(double)));
+            if (q < 3) HIPCHK(hipMemcpyAsync(buf[q], host[q], nb[q] * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        }
+        rc = project_impl(c, buf[0], buf[1], buf[2], buf[3], dt, max_iter, tol, st);
+        for (int q = 0; q < 4; q++)
+            if (host[q]) HIPCHK(hipMemcpyAsync(host[q], buf[q], nb[q] * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) { release(); return from_hip(c, e); }
+    catch (const std::bad_alloc&) { release(); return fail(c, UAAMG_ERR_OOM, "host allocation failed"); }
+    release();
     return rc;
 }
 
