@@ -221,6 +221,69 @@ def bench_batch(args):
     return res
 
 
+def bench_project(args):
+    """Pressure projection around the solve (PressureProjection3D::project,
+    pressure_3d.cpp:5-39) at 256^3: the plume's MAC velocities (the solve
+    benchmark's rhs is their divergence), dt = 1, FP32 relres 1e-6, every
+    array resident in HBM; each timed projection starts from the same
+    velocities (restored outside the timed call)."""
+    import torch
+    import uaamg_amd as U
+    from uaamg_amd import workloads as W
+    pb = W.plume3d(seed=0, n=args.n, velocity=True)
+    dev = torch.device("cuda:0")
+    u0, v0, w0 = (torch.from_numpy(a).to(dev) for a in (pb.u, pb.v, pb.w))
+    u, v, w = u0.clone(), v0.clone(), w0.clone()
+    p = torch.zeros(pb.N, dtype=torch.float64, device=dev)
+    ctx = U.Context(precision=U.FP32, tol_mode=U.TOL_REL, max_iter=1000, tol=1e-6, check_every=4)
+    ctx.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+
+    def once():
+        u.copy_(u0); v.copy_(v0); w.copy_(w0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = ctx.project_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), p.data_ptr(), 1.0, 1000, 1e-6)
+        return time.perf_counter() - t, st
+
+    for _ in range(max(1, args.warmup)):
+        once()
+    ts, its = [], []
+    for _ in range(args.steps):
+        dt_, st = once()
+        if st.status != 0:
+            raise RuntimeError(f"projection solve: {U.status_name(st.status)}")
+        ts.append(dt_)
+        its.append(st.iters)
+    # divergence before / after (fluid cells), on the device
+    n = pb.n[0]
+
+    def div(uu, vv, ww):
+        U3 = uu.view(n + 2, n + 2, n + 1)
+        V3 = vv.view(n + 2, n + 1, n + 2)
+        W3 = ww.view(n + 1, n + 2, n + 2)
+        d = (U3[1:-1, 1:-1, 1:] - U3[1:-1, 1:-1, :-1]) + (V3[1:-1, 1:, 1:-1] - V3[1:-1, :-1, 1:-1]) \
+            + (W3[1:, 1:-1, 1:-1] - W3[:-1, 1:-1, 1:-1])
+        return d * n
+    fl = torch.from_numpy(pb.solid.reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1] == 0).to(dev)
+    d0 = div(u0, v0, w0)[fl]
+    d1 = div(u, v, w)[fl]
+    r0 = float((d0 - d0.mean()).abs().max())
+    r1 = float((d1 - d1.mean()).abs().max())
+    ctx.close()
+    t = float(np.median(ts))
+    res = {
+        "workload": f"{pb.name} MAC velocities u = T curl(Psi) (the solve benchmark's rhs is their divergence), "
+                    "dt = 1, FP32 relres 1e-6: rhs = div(u)/dt, p = 0, solve, face correction",
+        "projections_per_s": round(1.0 / t, 3),
+        "ms_per_projection": round(1e3 * t, 3),
+        "iters": int(np.median(its)),
+        "max_div_fluid_before": r0,
+        "max_div_fluid_after": r1,
+    }
+    log(f"[project] {json.dumps(res)}")
+    return res
+
+
 def _serve_sims(nsim, seed, dev):
     """Simulations of the served stream: half 2D (side log-uniform 64..2048),
     half 3D (side log-uniform 32..256), all walls Neumann, up to 3 disjoint
@@ -428,6 +491,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--no-project", action="store_true", help="skip the projection measurement")
+    ap.add_argument("--project-only", action="store_true", help="only the projection measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (3x), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -443,6 +508,9 @@ def main():
         return
     if args.serve_only:
         print(json.dumps({"serving": bench_serve(args)}), flush=True)
+        return
+    if args.project_only:
+        print(json.dumps({"projection": bench_project(args)}), flush=True)
         return
 
     import torch
@@ -571,9 +639,12 @@ def main():
         },
         "cpu_baseline": None,
     }
-    if not args.no_batch:
+    if not args.no_project or not args.no_batch:
         del rhs_d, solid_d, p_d
         ctx.close()
+    if not args.no_project:
+        out["projection"] = bench_project(args)
+    if not args.no_batch:
         out["batched"] = bench_batch(args)
     if not args.no_serve:
         out["serving"] = bench_serve(args)

@@ -10,6 +10,8 @@
 //   CudaPCGSolver[3D]            include/solver/cuda_pcg_solver.h:14-32 (the
 //                                GPU drop-in precedent) -> HipPCGSolver[3D]
 //   amgpcg_solve                 test/solver/test_amgpcg.cpp:29-70
+//   PressureProjection[3D]       include/pressure/pressure.h, pressure_3d.h;
+//                                src/pressure/pressure.cpp:5-32, pressure_3d.cpp:5-39
 //
 // Behaviour kept from the reference: solve() writes g.p in place, adding the
 // PCG iterate on fluid interior cells (pcg.cpp:107); breakdown and
@@ -34,7 +36,11 @@ struct Mesh2D {
     int nx, ny;
     double dx, dy;
     Mesh2D(int nx_, int ny_, double lx, double ly) : nx(nx_), ny(ny_), dx(lx / nx_), dy(ly / ny_) {}
+    int u_size() const { return (nx + 1) * (ny + 2); }
+    int v_size() const { return (nx + 2) * (ny + 1); }
     int p_size() const { return (nx + 2) * (ny + 2); }
+    int iu(int i, int j) const { return i + j * (nx + 1); }
+    int iv(int i, int j) const { return i + j * (nx + 2); }
     int ip(int i, int j) const { return i + j * (nx + 2); }
 };
 
@@ -43,26 +49,39 @@ struct Mesh3D {
     double dx, dy, dz;
     Mesh3D(int nx_, int ny_, int nz_, double lx, double ly, double lz)
         : nx(nx_), ny(ny_), nz(nz_), dx(lx / nx_), dy(ly / ny_), dz(lz / nz_) {}
+    int u_size() const { return (nx + 1) * (ny + 2) * (nz + 2); }
+    int v_size() const { return (nx + 2) * (ny + 1) * (nz + 2); }
+    int w_size() const { return (nx + 2) * (ny + 2) * (nz + 1); }
     int p_size() const { return (nx + 2) * (ny + 2) * (nz + 2); }
+    int iu(int i, int j, int k) const { return i + j * (nx + 1) + k * (nx + 1) * (ny + 2); }
+    int iv(int i, int j, int k) const { return i + j * (nx + 2) + k * (nx + 2) * (ny + 1); }
+    int iw(int i, int j, int k) const { return i + j * (nx + 2) + k * (nx + 2) * (ny + 2); }
     int ip(int i, int j, int k) const { return i + j * (nx + 2) + k * (nx + 2) * (ny + 2); }
 };
 
 class Grid : public Mesh2D {
 public:
-    std::vector<double> p;
+    std::vector<double> u, v, p;
     std::vector<bool> solid;
-    Grid(int nx_, int ny_, double lx, double ly) : Mesh2D(nx_, ny_, lx, ly), p(p_size(), 0.0), solid(p_size(), false) {}
+    Grid(int nx_, int ny_, double lx, double ly)
+        : Mesh2D(nx_, ny_, lx, ly), u(u_size(), 0.0), v(v_size(), 0.0), p(p_size(), 0.0), solid(p_size(), false) {}
     bool is_solid(int i, int j) const { return solid[ip(i, j)]; }
     void set_solid(int i, int j) { solid[ip(i, j)] = true; }
+    double& u_at(int i, int j) { return u[iu(i, j)]; }
+    double& v_at(int i, int j) { return v[iv(i, j)]; }
 };
 
 struct Grid3D : public Mesh3D {
-    std::vector<double> p;
+    std::vector<double> u, v, w, p;
     std::vector<bool> solid;
     Grid3D(int nx_, int ny_, int nz_, double lx, double ly, double lz)
-        : Mesh3D(nx_, ny_, nz_, lx, ly, lz), p(p_size(), 0.0), solid(p_size(), false) {}
+        : Mesh3D(nx_, ny_, nz_, lx, ly, lz), u(u_size(), 0.0), v(v_size(), 0.0), w(w_size(), 0.0),
+          p(p_size(), 0.0), solid(p_size(), false) {}
     bool is_solid(int i, int j, int k) const { return solid[ip(i, j, k)]; }
     void set_solid(int i, int j, int k) { solid[ip(i, j, k)] = true; }
+    double& u_at(int i, int j, int k) { return u[iu(i, j, k)]; }
+    double& v_at(int i, int j, int k) { return v[iv(i, j, k)]; }
+    double& w_at(int i, int j, int k) { return w[iw(i, j, k)]; }
 };
 
 class Solver {
@@ -136,14 +155,18 @@ public:
         if ((int)rhs.size() != g.p_size()) throw std::invalid_argument("rhs size != (nx+2)(ny+2)");
         if (!ctx_) ctx_ = std::make_unique<detail::Ctx>(opts_);
         mask_.assign(g.solid.begin(), g.solid.end());
-        uaamg_grid gd{};
-        gd.dim = 2; gd.n[0] = g.nx; gd.n[1] = g.ny; gd.n[2] = 1;
-        gd.h[0] = g.dx; gd.h[1] = g.dy; gd.h[2] = 1.0;
-        gd.solid = mask_.data();
-        detail::check(ctx_->h, uaamg_set_grid(ctx_->h, &gd), "uaamg_set_grid");
+        set_grid(g);
         uaamg_stats st{};
         int rc = uaamg_solve(ctx_->h, rhs.data(), g.p.data(), max_iter, tol, &st);
         detail::check(ctx_->h, rc, "uaamg_solve");
+        last_ = st;
+    }
+    // rhs = div(u)/dt, p = 0, solve, u -= dt grad p, all on the device
+    void project(Grid& g, double dt, int max_iter, double tol) {
+        set_grid(g);
+        uaamg_stats st{};
+        int rc = uaamg_project(ctx_->h, g.u.data(), g.v.data(), nullptr, g.p.data(), dt, max_iter, tol, &st);
+        detail::check(ctx_->h, rc, "uaamg_project");
         last_ = st;
     }
     std::string name() const override { return "PCG(HIP-UAAMG)"; }
@@ -151,6 +174,15 @@ public:
     int last_status() const { return last_.status; }
 
 private:
+    void set_grid(const Grid& g) {
+        if (!ctx_) ctx_ = std::make_unique<detail::Ctx>(opts_);
+        mask_.assign(g.solid.begin(), g.solid.end());
+        uaamg_grid gd{};
+        gd.dim = 2; gd.n[0] = g.nx; gd.n[1] = g.ny; gd.n[2] = 1;
+        gd.h[0] = g.dx; gd.h[1] = g.dy; gd.h[2] = 1.0;
+        gd.solid = mask_.data();
+        detail::check(ctx_->h, uaamg_set_grid(ctx_->h, &gd), "uaamg_set_grid");
+    }
     uaamg_opts opts_;
     std::unique_ptr<detail::Ctx> ctx_;
     std::vector<uint8_t> mask_;
@@ -165,16 +197,18 @@ public:
     }
     void solve(Grid3D& g, const std::vector<double>& rhs, int max_iter, double tol) override {
         if ((int)rhs.size() != g.p_size()) throw std::invalid_argument("rhs size != (nx+2)(ny+2)(nz+2)");
-        if (!ctx_) ctx_ = std::make_unique<detail::Ctx>(opts_);
-        mask_.assign(g.solid.begin(), g.solid.end());
-        uaamg_grid gd{};
-        gd.dim = 3; gd.n[0] = g.nx; gd.n[1] = g.ny; gd.n[2] = g.nz;
-        gd.h[0] = g.dx; gd.h[1] = g.dy; gd.h[2] = g.dz;
-        gd.solid = mask_.data();
-        detail::check(ctx_->h, uaamg_set_grid(ctx_->h, &gd), "uaamg_set_grid");
+        set_grid(g);
         uaamg_stats st{};
         int rc = uaamg_solve(ctx_->h, rhs.data(), g.p.data(), max_iter, tol, &st);
         detail::check(ctx_->h, rc, "uaamg_solve");
+        last_ = st;
+    }
+    // rhs = div(u)/dt, p = 0, solve, u -= dt grad p, all on the device
+    void project(Grid3D& g, double dt, int max_iter, double tol) {
+        set_grid(g);
+        uaamg_stats st{};
+        int rc = uaamg_project(ctx_->h, g.u.data(), g.v.data(), g.w.data(), g.p.data(), dt, max_iter, tol, &st);
+        detail::check(ctx_->h, rc, "uaamg_project");
         last_ = st;
     }
     std::string name() const override { return "PCG3D(HIP-UAAMG3D)"; }
@@ -182,6 +216,15 @@ public:
     int last_status() const { return last_.status; }
 
 private:
+    void set_grid(const Grid3D& g) {
+        if (!ctx_) ctx_ = std::make_unique<detail::Ctx>(opts_);
+        mask_.assign(g.solid.begin(), g.solid.end());
+        uaamg_grid gd{};
+        gd.dim = 3; gd.n[0] = g.nx; gd.n[1] = g.ny; gd.n[2] = g.nz;
+        gd.h[0] = g.dx; gd.h[1] = g.dy; gd.h[2] = g.dz;
+        gd.solid = mask_.data();
+        detail::check(ctx_->h, uaamg_set_grid(ctx_->h, &gd), "uaamg_set_grid");
+    }
     uaamg_opts opts_;
     std::unique_ptr<detail::Ctx> ctx_;
     std::vector<uint8_t> mask_;
@@ -206,6 +249,24 @@ struct Factory3D {
         if (parse_solver_kind(key) == SolverKind::PCG_UAAMG && be == Backend::HIP)
             return std::make_unique<HipPCGSolver3D>(o);
         throw std::invalid_argument("solver key '" + key + "' has no back-end in this build");
+    }
+};
+
+// PressureProjection / PressureProjection3D::project (pressure.cpp:5-32,
+// pressure_3d.cpp:5-39): same signature; the solver must be the HIP one (its
+// context runs the rhs assembly, the solve and the correction on the device).
+struct PressureProjection {
+    static void project(Grid& g, double dt, Solver& solver, int max_iter, double tol) {
+        auto* s = dynamic_cast<HipPCGSolver*>(&solver);
+        if (!s) throw std::invalid_argument("PressureProjection::project needs HipPCGSolver in this build");
+        s->project(g, dt, max_iter, tol);
+    }
+};
+struct PressureProjection3D {
+    static void project(Grid3D& g, double dt, Solver3D& solver, int max_iter, double tol) {
+        auto* s = dynamic_cast<HipPCGSolver3D*>(&solver);
+        if (!s) throw std::invalid_argument("PressureProjection3D::project needs HipPCGSolver3D in this build");
+        s->project(g, dt, max_iter, tol);
     }
 };
 

@@ -96,6 +96,37 @@ static int run_solve(const std::string& out) {
         write_u8(out + "/g2_solid.u8", g.solid);
         write_f64(out + "/g2_p1.f64", g.p);
     }
+    // projection, 3D: the grid of the first case with a smooth velocity field
+    {
+        Grid3D g(24, 16, 12, 1.5, 1.0, 0.75);
+        for (int k = 1; k <= g.nz; ++k)
+            for (int j = 1; j <= g.ny; ++j)
+                for (int i = 1; i <= g.nx; ++i) {
+                    double dx = i - 16.0, dy = j - 8.0, dz = k - 6.0;
+                    if (dx * dx + dy * dy + dz * dz < 9.0) g.set_solid(i, j, k);
+                }
+        for (int k = 0; k < g.nz + 2; ++k)
+            for (int j = 0; j < g.ny + 2; ++j)
+                for (int i = 0; i <= g.nx; ++i) g.u_at(i, j, k) = std::sin(0.4 * i + 0.3 * j) * std::cos(0.2 * k);
+        for (int k = 0; k < g.nz + 2; ++k)
+            for (int j = 0; j <= g.ny; ++j)
+                for (int i = 0; i < g.nx + 2; ++i) g.v_at(i, j, k) = std::cos(0.25 * i) * std::sin(0.5 * j + 0.1 * k);
+        for (int k = 0; k <= g.nz; ++k)
+            for (int j = 0; j < g.ny + 2; ++j)
+                for (int i = 0; i < g.nx + 2; ++i) g.w_at(i, j, k) = std::sin(0.15 * i * k + 0.2 * j);
+        write_f64(out + "/pj_u0.f64", g.u);
+        write_f64(out + "/pj_v0.f64", g.v);
+        write_f64(out + "/pj_w0.f64", g.w);
+        write_u8(out + "/pj_solid.u8", g.solid);
+        auto s = Factory3D::create("pcg_uaamg", Backend::HIP);   // reference options: FP64, absolute tol
+        PressureProjection3D::project(g, 0.01, *s, 300, 1e-8);
+        auto* hs = dynamic_cast<HipPCGSolver3D*>(s.get());
+        std::printf("projection iters %d status %d\n", hs->last_stats().iters, hs->last_status());
+        write_f64(out + "/pj_u.f64", g.u);
+        write_f64(out + "/pj_v.f64", g.v);
+        write_f64(out + "/pj_w.f64", g.w);
+        write_f64(out + "/pj_p.f64", g.p);
+    }
     // standalone amgpcg, N = 64, b = sin(pi x) sin(pi y) (test_amgpcg.cpp:29-70)
     {
         amg::Grid b(64);

@@ -69,3 +69,16 @@ def test_host_mirror_amgpcg_matches_oracle(host_run):
     x = _rd(d, "amg_x.f64")
     assert int(_rd(d, "amg_iters.f64")[0]) == it
     assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+
+
+def test_host_mirror_projection_matches_oracle(host_run):
+    """PressureProjection3D::project through the host mirror (FP64, the
+    reference's options) equals the oracle's projection."""
+    d, _ = host_run
+    pb = ob.Problem((24, 16, 12), (1.5, 1.0, 0.75), solid=_rd(d, "pj_solid.u8", np.uint8))
+    u0, v0, w0 = _rd(d, "pj_u0.f64"), _rd(d, "pj_v0.f64"), _rd(d, "pj_w0.f64")
+    uo, vo, wo, po, res = ob.project(pb, u0, v0, w0, 0.01, 300, 1e-8)
+    assert np.abs(_rd(d, "pj_p.f64") - po).max() <= 1e-9 * np.abs(po).max()
+    vmax = max(np.abs(u0).max(), np.abs(v0).max(), np.abs(w0).max())
+    for name, ref in (("pj_u.f64", uo), ("pj_v.f64", vo), ("pj_w.f64", wo)):
+        assert np.abs(_rd(d, name) - ref).max() <= 1e-9 * vmax

@@ -118,11 +118,13 @@ def _upsample_trilinear(coarse, n):
     return out
 
 
-def plume3d(seed=0, n=256, nspheres=8):
+def plume3d(seed=0, n=256, nspheres=8, velocity=False):
     """n^3 cells, dx = 1/n, Neumann walls; solid floor slab y < 0.1 and
     `nspheres` spheres (r ~ U(0.03, 0.1), centres ~ U(0,1)^3); velocity
     T(x) curl(Psi), T a Gaussian temperature blob, Psi a 16^3 random vector
-    field trilinearly upsampled; rhs = div(u) on fluid cells, zero mean."""
+    field trilinearly upsampled; rhs = div(u) on fluid cells, zero mean.
+    velocity: also keep the MAC face velocities as pb.u, pb.v, pb.w in the
+    reference layouts (mesh_3d.h:29-37), ghost rows zero."""
     rng = np.random.default_rng(seed)
     h = 1.0 / n
     c = _centres(n, h)
@@ -150,14 +152,24 @@ def plume3d(seed=0, n=256, nspheres=8):
     ux = np.zeros((n, n, n + 1)); ux[:, :, 1:-1] = 0.5 * (vx[:, :, 1:] + vx[:, :, :-1])
     ux[:, :, 1:-1][sol_in[:, :, 1:] | sol_in[:, :, :-1]] = 0.0
     div = (ux[:, :, 1:] - ux[:, :, :-1]) / h
+    mac = {}
+    if velocity:
+        U = np.zeros((n + 2, n + 2, n + 1)); U[1:-1, 1:-1, :] = ux
+        mac["u"] = U.reshape(-1)
     del ux, vx
     uy = np.zeros((n, n + 1, n)); uy[:, 1:-1, :] = 0.5 * (vy[:, 1:, :] + vy[:, :-1, :])
     uy[:, 1:-1, :][sol_in[:, 1:, :] | sol_in[:, :-1, :]] = 0.0
     div += (uy[:, 1:, :] - uy[:, :-1, :]) / h
+    if velocity:
+        V = np.zeros((n + 2, n + 1, n + 2)); V[1:-1, :, 1:-1] = uy
+        mac["v"] = V.reshape(-1)
     del uy, vy
     uz = np.zeros((n + 1, n, n)); uz[1:-1, :, :] = 0.5 * (vz[1:, :, :] + vz[:-1, :, :])
     uz[1:-1, :, :][sol_in[1:, :, :] | sol_in[:-1, :, :]] = 0.0
     div += (uz[1:, :, :] - uz[:-1, :, :]) / h
+    if velocity:
+        W = np.zeros((n + 1, n + 2, n + 2)); W[:, 1:-1, 1:-1] = uz
+        mac["w"] = W.reshape(-1)
     del uz, vz
     rhs = np.zeros((n + 2, n + 2, n + 2))
     rhs[1:-1, 1:-1, 1:-1] = div
@@ -165,6 +177,8 @@ def plume3d(seed=0, n=256, nspheres=8):
     pb = Problem((n, n, n), (1.0, 1.0, 1.0), solid.astype(np.uint8).reshape(-1), rhs.reshape(-1),
                  name=f"plume3d_{n}^3")
     _zero_mean(pb.rhs, pb.fluid_mask())
+    for k, a in mac.items():
+        setattr(pb, k, a)
     return pb
 
 
