@@ -114,14 +114,14 @@ public:
         kb_ctrl_init<S><<<(G + 63) / 64, 64, 0, s_>>>(dd_, G, max_iter, tol, o_.tol_mode, o_.flexible, o_.project_mean);
         kb_cast_warm<D, S><<<nflat_, 256, 0, s_>>>(dd_, flat_);
         kb_init_sum<S><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-        kb_fin_init_sum<S><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+        kb_fin_init_sum<S><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         kb_init_r<S><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-        kb_fin_init_r<S><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+        kb_fin_init_r<S><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         kb_init_sub<S><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-        kb_fin_init_sub<S><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+        kb_fin_init_sub<S><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         if (any_warm_) {
             kb_init_warm<D, S><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-            kb_fin_init_warm<S><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+            kb_fin_init_warm<S><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         }
         vcycle();
         kb_pupdate<S><<<nflat_, 256, 0, s_>>>(dd_, flat_);
@@ -221,13 +221,13 @@ private:
     void iteration() {
         const int G = size();
         kb_matvec<D, S><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-        kb_fin_matvec<S><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+        kb_fin_matvec<S><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         if (o_.flexible) {
             kb_update<S, true><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-            kb_fin_update<S, true><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+            kb_fin_update<S, true><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         } else {
             kb_update<S, false><<<nflat_, 256, 0, s_>>>(dd_, flat_, pflat_);
-            kb_fin_update<S, false><<<G, 1024, 0, s_>>>(dd_, pflat_, nflat_);
+            kb_fin_update<S, false><<<G, FIN_THREADS, 0, s_>>>(dd_, pflat_, nflat_);
         }
         vcycle();
         kb_pupdate<S><<<nflat_, 256, 0, s_>>>(dd_, flat_);
@@ -249,7 +249,7 @@ private:
             if (nrows_[1]) kb_rows_up0<4, S><<<nrows_[1], 640, 0, s_>>>(dd_, rows_[1], prows_[1]);
         }
         if (nm0_) kb_march_up0<D, RX0, RY0, PPT0, S><<<nm0_, 256, 0, s_>>>(dd_, m0_, pm0_);
-        kb_fin_zsums<S><<<G, 1024, 0, s_>>>(dd_);
+        kb_fin_zsums<S><<<G, FIN_THREADS, 0, s_>>>(dd_);
     }
 
     // ---- descriptors and work-lists ----

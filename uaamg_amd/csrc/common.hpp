@@ -108,6 +108,11 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
+// Workgroup size of the one-workgroup finish kernels: partial counts are a
+// few hundred to 2048, so four waves load them in a few rounds and the
+// block-level sum crosses four waves instead of sixteen.
+constexpr int FIN_THREADS = 256;
+
 // Deterministic two-kernel reduction.  Producer kernels call block_store():
 // each block writes its NV partial sums to partials[q * nblocks + block].  A
 // one-workgroup finish kernel then adds the partials in block order

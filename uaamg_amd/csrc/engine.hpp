@@ -210,7 +210,7 @@ public:
         const int nl = nlevels();
         HIPCHK(hipMemcpyAsync(solid0_, solid_dev, g0_.N, hipMemcpyDeviceToDevice, stream_));
         k_nbr0<D><<<lc0_.grid, lc0_.block, 0, stream_>>>(solid0_, nbr0_, g0_, lc0_.kz, partials_);
-        k_fin_count<<<1, 1024, 0, stream_>>>(partials_, (int)lc0_.nblocks(), count_);
+        k_fin_count<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)lc0_.nblocks(), count_);
         HIPCHK(hipGetLastError());
         for (int l = 1; l < nl; l++) {
             Level& F = lv_[l - 1]; Level& C = lv_[l];
@@ -501,7 +501,7 @@ public:
                 tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S> : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
                         rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const Ctrl*)ctrl_);
-                k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+                k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
                 return;
             }
             tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0<4, ROWS_NW, ROWS_RW, S> : k_rows_down0<2, ROWS_NW, ROWS_RW, S>,
@@ -512,7 +512,7 @@ public:
         if constexpr (D == 3 && std::is_same<T, float>::value) {
             tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? k_rows_up0<4, ROWS_NW, ROWS_RW, S> : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
-            k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+            k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
         }
     }
 
@@ -571,7 +571,7 @@ public:
                 MLaunch m = march_launch(g0_, RX0, RY0, 1024);
                 tlaunch(timer_, UAAMG_K_UP0, k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S>, m.grid, dim3(256), stream_,
                         r, A0, (const S*)C.x, C.g, z, m.KC, skip, partials_);
-                k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
+                k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
             } else if (rcpl_[l]) {
                 TScope ts(timer_, UAAMG_K_COARSE);
                 rows_upc(l, skip);
@@ -616,14 +616,14 @@ public:
             fr_old_ = nullptr;
             const unsigned fb = flat_blocks(2048);
             k_init_sum<<<fb, 256, 0, stream_>>>(rhs_dev, nbr0_, N4_, partials_, ctrl_);
-            k_fin_init_sum<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_fin_init_sum<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             k_init_r<T><<<fb, 256, 0, stream_>>>(rhs_dev, r_, nbr0_, N4_, partials_, ctrl_);
-            k_fin_init_r<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_fin_init_r<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             k_init_sub<T><<<fb, 256, 0, stream_>>>(r_, nbr0_, N4_, partials_, ctrl_);
-            k_fin_init_sub<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_fin_init_sub<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             if (warm) {
                 k_init_warm<D, T><<<fb, 256, 0, stream_>>>(r_, x, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
-                k_fin_init_warm<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+                k_fin_init_warm<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             }
             HIPCHK(hipGetLastError());
         }
@@ -765,7 +765,7 @@ public:
             tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
                     (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
         }
-        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)mb, ctrl_);
+        k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)mb, ctrl_);
         if (fr_) {   // r update and convergence test inside the level-0 down kernel
             fr_old_ = rbuf(pcur_);
             T* rnew = rbuf(pcur_ ^ 1);
@@ -779,11 +779,11 @@ public:
         if (o_.flexible) {
             tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, true>, dim3(fb), dim3(256), stream_,
                     x, r_, pv, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
-            k_fin_update<true><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_fin_update<true><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
         } else {
             tlaunch(timer_, UAAMG_K_UPDATE, k_update<T, false>, dim3(fb), dim3(256), stream_,
                     x, r_, pv, (const T*)Ap_, (const T*)z_, (const uint8_t*)nbr0_, N4_, partials_, ctrl_);
-            k_fin_update<false><<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+            k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
         }
         vcycle(r_, z_, skip);
         zsums(skip);
@@ -795,7 +795,7 @@ public:
         TScope ts(timer_, UAAMG_K_PUPDATE);
         const unsigned fb = flat_blocks(2048);
         k_zsums<T><<<fb, 256, 0, stream_>>>(z_, r_, nbr0_, N4_, partials_, ctrl_);
-        k_fin_zsums<<<1, 1024, 0, stream_>>>(partials_, (int)fb, ctrl_);
+        k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
     }
     void pupdate() {
         tlaunch(timer_, UAAMG_K_PUPDATE, k_pupdate<T>, dim3(flat_blocks(2048)), dim3(256), stream_,
@@ -815,7 +815,7 @@ public:
         k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, 1, 0.0, 0, project_, 0, nullptr);
         cast_in_all(v_dev, p_);
         k_matvec_dot<D, T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
-        k_fin_matvec<<<1, 1024, 0, stream_>>>(partials_, (int)flat_blocks(2048), ctrl_);
+        k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)flat_blocks(2048), ctrl_);
         cast_out_all(Ap_, Av_dev);
         HIPCHK(hipMemcpyAsync(ctrl_host_, ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));

@@ -201,8 +201,26 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restric
     auto level_coarsest = [&](auto& P, int l) {
         CT* X = lds + A.xo[l];
         const CT* Bv = lds + A.bo[l];
+        const Geo& g = A.g[l];
+        const int npairs = ((g.nx + 1) >> 1) * g.ny * (D == 3 ? g.nz : 1);
+        if (npairs <= 64) {
+            // every pair is owned by a lane of wave 0 (thread = pair): the
+            // passes need no workgroup barrier, only wave order (LDS accesses
+            // of one wave complete in program order; the fences keep the
+            // compiler from moving them across passes)
+            if (tid < 64) {
+                for (int s = 0; s < A.npass; s++) {
+                    tail_smooth<D>(P, X, Bv, g, (s & 1) ? 0 : 1);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            __syncthreads();
+            return;
+        }
         for (int s = 0; s < A.npass; s++) {     // odd, even, odd, ... (forward/reverse pairs collapsed)
-            tail_smooth<D>(P, X, Bv, A.g[l], (s & 1) ? 0 : 1);
+            tail_smooth<D>(P, X, Bv, g, (s & 1) ? 0 : 1);
             __syncthreads();
         }
     };
