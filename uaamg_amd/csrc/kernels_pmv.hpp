@@ -9,11 +9,11 @@
 //                                            neighbour byte + Dirichlet excess)
 //   partial sums of p_new . Ap             (deterministic, per workgroup)
 // and, with XL, the x update of the previous iteration, x += alpha p_old
-// (k_update's formula; alpha = 0 before the first operator), so the CG
-// update itself only has to touch r (fused into the level-0 down kernel).
+// (alpha = 0 before the first operator), so the CG update itself only has to
+// touch r (fused into the level-0 down kernel).
 // p_old and p_new are separate buffers: a workgroup recomputes p_new on its
 // halo row and planes from z and p_old, so no workgroup reads what another
-// one writes.  p_new is k_pupdate's formula; the operator is k_matvec_dot's
+// one writes.  p_new is k_pupdate's formula (in FP32); the operator is k_matvec_dot's
 // (apply_A) in the stencil form of the row kernels: p_new is 0 on every
 // non-fluid cell, so a face to a solid cell or a Neumann wall drops out of
 // both d and the sum, and a Dirichlet wall face adds 2/h^2 to d (apply_A's
@@ -65,7 +65,10 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     RowThread<CPL, NW, RW, 1> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
-    const double mean = ctrl->mean_z, beta = ctrl->beta, alpha = XL ? ctrl->alpha : 0.0;
+    // the vector updates run in FP32 (one FMA each) with the FP64 scalars
+    // rounded once: they differ from k_pupdate / k_update's FP64 forms by the
+    // rounding of the scalar, far below the FP32 storage of the vectors
+    const float mean = (float)ctrl->mean_z, beta = (float)ctrl->beta, alpha = XL ? (float)ctrl->alpha : 0.0f;
 
     if (active) {
         float PN[4][RW][CPL];        // p_new of planes (slot = plane & 3)
@@ -110,7 +113,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     const uint8_t nb = pin ? pfn[PB][t][q] : (uint8_t)0;
                     NB[S][t][q] = nb;
                     PN[S][t][q] = (nb & NB_FLUID)
-                        ? (float)(((double)pfz[PB][t][q] - mean) + beta * (double)pfp[PB][t][q]) : 0.0f;
+                        ? fmaf(beta, pfp[PB][t][q], pfz[PB][t][q] - mean) : 0.0f;
                 }
             if constexpr (XL) {   // x += alpha p_old on the tile's cells of plane P
                 if (P >= th.m0 && P <= th.last) {
@@ -120,8 +123,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                         float xn[CPL];
 #pragma unroll
                         for (int q = 0; q < CPL; q++)
-                            xn[q] = (NB[S][t][q] & NB_FLUID) ? (float)((double)pfx[PB][t][q] + alpha * (double)pfp[PB][t][q])
-                                                             : pfx[PB][t][q];
+                            xn[q] = (NB[S][t][q] & NB_FLUID) ? fmaf(alpha, pfp[PB][t][q], pfx[PB][t][q]) : pfx[PB][t][q];
                         const int id = th.roff[t] + P * (int)g.sy;
                         if (th.okx[CPL - 1]) {
                             typename RowVec<CPL>::V v;

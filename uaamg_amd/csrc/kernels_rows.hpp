@@ -209,7 +209,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         return;
     }
     const int ty = tile % rg.nty, kc = tile / rg.nty;
-    const double alpha = FR ? u.ctrl->alpha : 0.0;
+    const float alpha = FR ? (float)u.ctrl->alpha : 0.0f;   // r update in FP32 (one FMA), as in k_rows_pmv
     double rr[1] = {0.0};
     RowThread<CPL, NW, RW> th;
     th.init(g, rg, ty, kc);
@@ -267,7 +267,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
                 sfor<CPL>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     const bool fl = pfn[PB][t][q] & NB_FLUID;
-                    const float rn = fl ? (float)((double)pfr[PB][t][q] - alpha * (double)pfa[PB][t][q]) : 0.0f;
+                    const float rn = fl ? fmaf(-alpha, pfa[PB][t][q], pfr[PB][t][q]) : 0.0f;
                     pfr[PB][t][q] = rn;
                     rr[0] += (fl && pin && th.tile_row[t] && th.okx[q]) ? (double)rn * (double)rn : 0.0;
                 });
