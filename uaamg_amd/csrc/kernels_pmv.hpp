@@ -152,6 +152,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     const float el = lane_from_left(PN[S1][t][CPL - 1]);
                     const float er = lane_from_right(PN[S1][t][0]);
                     float out[CPL];
+                    float pap = 0.0f;   // the row's p.Ap in FP32, accumulated in FP64
                     sfor<CPL>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
                         const float vc = PN[S1][t][q];
@@ -175,9 +176,10 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                                          + K.idz2 * ((double)PN[S2][t][q] + (double)PN[S][t][q]);
                         const bool fl = nb & NB_FLUID;
                         const float o = fl ? (float)(d * (double)vc - sum) : 0.0f;
-                        acc[0] += (fl && th.tile_row[t] && th.okx[q]) ? (double)vc * (double)o : 0.0;
+                        pap = fmaf(th.okx[q] ? vc : 0.0f, o, pap);
                         out[q] = o;
                     });
+                    if (th.tile_row[t]) acc[0] += (double)pap;
                     if (th.tile_row[t]) {
                         const int id = th.roff[t] + R * (int)g.sy;
                         if (th.okx[CPL - 1]) {

@@ -264,13 +264,15 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
             const bool pin = P >= th.m0 && P <= th.last;
             sfor<RW>([&](auto tc) {
                 constexpr int t = decltype(tc)::value;
+                float rs = 0.0f;   // the row's r.r in FP32, accumulated in FP64
                 sfor<CPL>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
                     const bool fl = pfn[PB][t][q] & NB_FLUID;
                     const float rn = fl ? fmaf(-alpha, pfa[PB][t][q], pfr[PB][t][q]) : 0.0f;
                     pfr[PB][t][q] = rn;
-                    rr[0] += (fl && pin && th.tile_row[t] && th.okx[q]) ? (double)rn * (double)rn : 0.0;
+                    rs = fmaf(th.okx[q] ? rn : 0.0f, rn, rs);
                 });
+                if (pin && th.tile_row[t]) rr[0] += (double)rs;
                 if (pin && th.tile_row[t]) {
                     const int id = th.roff[t] + P * (int)g.sy;
                     if (th.okx[CPL - 1]) {
@@ -528,17 +530,30 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
                         }
                     });
                     if (th.tile_row[t]) {
+                        // the row's CPL cells summed in FP32 (one FMA each), the
+                        // row sums accumulated in FP64
+                        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
                         sfor<CPL>([&](auto qc) {
                             constexpr int q = decltype(qc)::value;
-                            if (th.okx[q]) {
-                                const float zv = X[S2][t][q];
-                                z[th.roff[t] + q + R * (int)g.sy] = zv;
-                                const double zz = (double)zv, rr = (double)Rq[S2][t][q];
-                                red[0] += zz;
-                                red[1] += rr * zz;
-                                red[2] += rr;
-                            }
+                            const float zv = th.okx[q] ? X[S2][t][q] : 0.0f, rv = th.okx[q] ? Rq[S2][t][q] : 0.0f;
+                            s0 += zv;
+                            s1 = fmaf(rv, zv, s1);
+                            s2 += rv;
                         });
+                        red[0] += (double)s0;
+                        red[1] += (double)s1;
+                        red[2] += (double)s2;
+                        const int id = th.roff[t] + R * (int)g.sy;
+                        if (th.okx[CPL - 1]) {
+                            typename RowVec<CPL>::V v;
+#pragma unroll
+                            for (int q = 0; q < CPL; q++) v[q] = X[S2][t][q];
+                            *(typename RowVec<CPL>::V*)(z + id) = v;
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < CPL; q++)
+                                if (th.okx[q]) z[id + q] = X[S2][t][q];
+                        }
                     }
                 });
             }
