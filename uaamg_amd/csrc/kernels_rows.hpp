@@ -925,9 +925,17 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
 #pragma unroll
             for (int t = 0; t < RW; t++) {
                 if (!th.tile_row[t]) continue;
+                const int id = th.roff[t] + R * (int)g.sy;
+                if (std::is_same<SV, float>::value && th.okx[CPL - 1]) {
+                    typename RowVec<CPL>::V v;
 #pragma unroll
-                for (int q = 0; q < CPL; q++)
-                    if (th.okx[q]) xout[th.roff[t] + q + R * (int)g.sy] = (SV)Q.X[S2][t][q];
+                    for (int q = 0; q < CPL; q++) v[q] = Q.X[S2][t][q];
+                    *(typename RowVec<CPL>::V*)((float*)xout + id) = v;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < CPL; q++)
+                        if (th.okx[q]) xout[id + q] = (SV)Q.X[S2][t][q];
+                }
             }
         }
         __syncthreads();
