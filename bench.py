@@ -37,8 +37,7 @@ def per_cell_bytes(cls, sv, sl, dim, mode):
     class (each array read or written once; nbr = 1-byte neighbour mask).
     mode "plain": operator, CG update, down, up, p update as five kernels;
     "pmv": the p update fused into the operator pass; "split": in addition the
-    x update rides on the next operator pass and the r update on the down
-    kernel (no separate update kernel)."""
+    r update rides on the down kernel and the update kernel only does x."""
     rho = 1.0 / 8 if dim == 3 else 1.0 / 4
     table = {
         "matvec_dot": 2 * sv + 1,          # read p, nbr; write Ap
@@ -50,7 +49,7 @@ def per_cell_bytes(cls, sv, sl, dim, mode):
     if mode in ("pmv", "split"):
         table["matvec_dot"] = 4 * sv + 1   # read z, p_old, nbr; write p_new, Ap
     if mode == "split":
-        table["matvec_dot"] += 2 * sv      # + read / write x
+        table["update"] = 3 * sv + 1       # read x, p, nbr; write x
         table["down0"] = 3 * sv + 1 + rho * sl   # read r_old, Ap, nbr; write r_new, coarse b
     return table.get(cls)
 
@@ -585,11 +584,7 @@ def main():
             log(f"[bench] {KNAMES[c]:>10s}: {ms_k[c] / n_k[c] * 1e3:8.1f} us/launch x {int(n_k[c] / args.steps)} per solve")
     sv = 8 if prec == U.FP64 else 4
     sl = {U.FP64: 8, U.FP32: 4}.get(prec, 2)
-    # which level-0 kernel set ran: no p-update launches = fused into the
-    # operator pass; no update launches either = split update
-    mode = "plain"
-    if n_k[U.capi.K_PUPDATE] == 0:
-        mode = "split" if n_k[U.capi.K_UPDATE] == 0 else "pmv"
+    mode = {0: "plain", 1: "pmv", 2: "split"}[st.kernel_set]   # the level-0 kernel set the solves ran
     cand = [c for c in range(5) if n_k[c] > 0]
     dom = max(cand, key=lambda c: ms_k[c])
     dname = KNAMES[dom]
@@ -626,7 +621,8 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": {"matvec_dot": "p update + operator (k_rows_pmv)" if mode != "plain" else "operator (k_matvec_dot)",
-                       "update": "CG update (k_update)", "down0": "level-0 down (k_rows_down0" + ("r)" if mode == "split" else ")"),
+                       "update": "x update (k_xupdate, second stream)" if mode == "split" else "CG update (k_update)",
+                       "down0": "level-0 down (k_rows_down0" + ("r)" if mode == "split" else ")"),
                        "up0": "level-0 up (k_rows_up0)", "pupdate": "p update (k_pupdate)"}[dname],
             "kernel_mode": mode,
             "achieved": round(achieved, 1),

@@ -78,6 +78,9 @@ typedef struct {
     double ms_kernel[8];       /* device time per kernel class timed (opts.timing bit set), see UAAMG_K_* */
     int    n_kernel[8];        /* launches per timed class */
     double alg_bytes;          /* canonical algorithmic HBM bytes of this solve */
+    int    kernel_set;         /* level-0 CG kernels used: 0 separate passes; 1 p update fused into the
+                                  operator; 2 in addition r update in the level-0 down kernel and the
+                                  x update on a second stream, beside the coarse levels */
 } uaamg_stats;
 
 /* kernel classes for uaamg_stats.ms_kernel */

@@ -55,7 +55,7 @@ int main(void) {
   F(uaamg_opts, timing) F(uaamg_opts, flexible)
   F(uaamg_stats, iters) F(uaamg_stats, status) F(uaamg_stats, nlevels) F(uaamg_stats, rnorm)
   F(uaamg_stats, r0norm) F(uaamg_stats, ms_total) F(uaamg_stats, ms_setup) F(uaamg_stats, ms_kernel)
-  F(uaamg_stats, n_kernel) F(uaamg_stats, alg_bytes)
+  F(uaamg_stats, n_kernel) F(uaamg_stats, alg_bytes) F(uaamg_stats, kernel_set)
   return 0;
 }
 """

@@ -36,7 +36,7 @@ class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("status", C.c_int), ("nlevels", C.c_int),
                 ("rnorm", C.c_double), ("r0norm", C.c_double), ("ms_total", C.c_double),
                 ("ms_setup", C.c_double), ("ms_kernel", C.c_double * 8), ("n_kernel", C.c_int * 8),
-                ("alg_bytes", C.c_double)]
+                ("alg_bytes", C.c_double), ("kernel_set", C.c_int)]
 
 
 EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",

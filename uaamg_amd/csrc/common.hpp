@@ -51,6 +51,7 @@ struct Ctrl {
     double m1, m2, count;
     double sum_r, sum_z, sum_rz;
     double rz_cross;       // (r_new, z_old - mean_old) for the flexible beta
+    double xalpha;         // alpha of this iteration's x update; 0 when the operator finish found the solve stopped
     int iter, max_iter, done, status, first, tol_mode, project, flexible;
     double* hist;          // optional per-iteration r.r history (nullptr: off)
 };

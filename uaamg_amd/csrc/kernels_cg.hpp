@@ -265,7 +265,7 @@ __global__ void k_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const 
     d_matvec_dot<D, T>(p, Ap, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x);
 }
 __device__ __forceinline__ void d_fin_matvec(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
-    if (ctrl->done) return;
+    if (ctrl->done) { if (threadIdx.x == 0) ctrl->xalpha = 0.0; return; }
     double tot[1];
     reduce_partials_at<1>(partials, nbt, base, nb, tot);
     if (threadIdx.x == 0) {
@@ -275,6 +275,7 @@ __device__ __forceinline__ void d_fin_matvec(const double* partials, int nbt, in
         if (!isfinite(pAp)) { ctrl->done = 1; ctrl->status = ST_NONFINITE; ctrl->iter += 1; ctrl->alpha = 0.0; }
         else if (pAp < 1e-15) { ctrl->done = 1; ctrl->status = ST_BREAKDOWN; ctrl->iter += 1; ctrl->alpha = 0.0; }
         else ctrl->alpha = ctrl->rsold / pAp;
+        ctrl->xalpha = ctrl->alpha;
     }
 }
 __global__ void k_fin_matvec(const double* partials, int nb, Ctrl* ctrl) {
@@ -438,6 +439,27 @@ __device__ __forceinline__ void d_fin_zsums(const double* partials, int nbt, int
 }
 __global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
     d_fin_zsums(partials, nb, 0, nb, ctrl);
+}
+
+// x += alpha p on fluid cells, alpha = ctrl->xalpha (the x half of k_update,
+// launched on its own; it does not read the done flag, which the r half may
+// set while it runs).
+template <typename T>
+__global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint8_t* __restrict__ nbr, long N4,
+                          const Ctrl* ctrl) {
+    const double alpha = ctrl->xalpha;
+    if (alpha == 0.0) return;
+    const int fv_b = blockIdx.x, fv_n = gridDim.x;
+    FLAT_LOOP(N4) {
+        long c = 4 * v_;
+        uint32_t w = *(const uint32_t*)(nbr + c);
+        if (!(w & 0x01010101u)) continue;
+        vec4<T> xv = *(const vec4<T>*)(x + c), pv = *(const vec4<T>*)(p + c);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (nbyte(w, q) & NB_FLUID) xv[q] = (T)((double)xv[q] + alpha * (double)pv[q]);
+        *(vec4<T>*)(x + c) = xv;
+    }
 }
 
 // p = (z - mean) + beta p on fluid cells.

@@ -8,9 +8,6 @@
 //                                            values, rounded once; d from the
 //                                            neighbour byte + Dirichlet excess)
 //   partial sums of p_new . Ap             (deterministic, per workgroup)
-// and, with XL, the x update of the previous iteration, x += alpha p_old
-// (alpha = 0 before the first operator), so the CG update itself only has to
-// touch r (fused into the level-0 down kernel).
 // p_old and p_new are separate buffers: a workgroup recomputes p_new on its
 // halo row and planes from z and p_old, so no workgroup reads what another
 // one writes.  p_new is k_pupdate's formula (in FP32); the operator is k_matvec_dot's
@@ -48,12 +45,11 @@ struct PmvLDS {
     float ex[2][NW][2][CPL][64];   // [step parity][wave][first / last row][cell][lane]
 };
 
-template <int CPL, int NW, int RW, bool XL>
+template <int CPL, int NW, int RW>
 __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                            float* __restrict__ pnew, float* __restrict__ Ap,
                                            const uint8_t* __restrict__ nbr, const Geo& g, const PmvTab& K,
-                                           const RowGeo& rg, double* partials, const Ctrl* ctrl, int tile,
-                                           float* __restrict__ x) {
+                                           const RowGeo& rg, double* partials, const Ctrl* ctrl, int tile) {
     if (ctrl->done) return;
     __shared__ PmvLDS<CPL, NW> L;
     __shared__ double sd[64];
@@ -65,15 +61,15 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     RowThread<CPL, NW, RW, 1> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
-    // the vector updates run in FP32 (one FMA each) with the FP64 scalars
-    // rounded once: they differ from k_pupdate / k_update's FP64 forms by the
-    // rounding of the scalar, far below the FP32 storage of the vectors
-    const float mean = (float)ctrl->mean_z, beta = (float)ctrl->beta, alpha = XL ? (float)ctrl->alpha : 0.0f;
+    // the p update runs in FP32 (one FMA) with the FP64 scalars rounded once:
+    // it differs from k_pupdate's FP64 form by the rounding of the scalars,
+    // far below the FP32 storage of the vectors
+    const float mean = (float)ctrl->mean_z, beta = (float)ctrl->beta;
 
     if (active) {
         float PN[4][RW][CPL];        // p_new of planes (slot = plane & 3)
         uint8_t NB[4][RW][CPL];
-        float pfz[2][RW][CPL], pfp[2][RW][CPL], pfx[XL ? 2 : 1][RW][CPL];
+        float pfz[2][RW][CPL], pfp[2][RW][CPL];
         uint8_t pfn[2][RW][CPL];
 #pragma unroll
         for (int s = 0; s < 4; s++)
@@ -90,11 +86,6 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                 const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
 #pragma unroll
                 for (int q = 0; q < CPL; q++) pfp[slot][t][q] = v[q];
-                if constexpr (XL) {
-                    const typename RowVec<CPL>::V xv = *(const typename RowVec<CPL>::V*)(x + th.roff[t] + po);
-#pragma unroll
-                    for (int q = 0; q < CPL; q++) pfx[slot][t][q] = xv[q];
-                }
             }
         };
         issue(0, th.m0 - 1);
@@ -115,29 +106,6 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     PN[S][t][q] = (nb & NB_FLUID)
                         ? fmaf(beta, pfp[PB][t][q], pfz[PB][t][q] - mean) : 0.0f;
                 }
-            if constexpr (XL) {   // x += alpha p_old on the tile's cells of plane P
-                if (P >= th.m0 && P <= th.last) {
-#pragma unroll
-                    for (int t = 0; t < RW; t++) {
-                        if (!th.tile_row[t]) continue;
-                        float xn[CPL];
-#pragma unroll
-                        for (int q = 0; q < CPL; q++)
-                            xn[q] = (NB[S][t][q] & NB_FLUID) ? fmaf(alpha, pfp[PB][t][q], pfx[PB][t][q]) : pfx[PB][t][q];
-                        const int id = th.roff[t] + P * (int)g.sy;
-                        if (th.okx[CPL - 1]) {
-                            typename RowVec<CPL>::V v;
-#pragma unroll
-                            for (int q = 0; q < CPL; q++) v[q] = xn[q];
-                            *(typename RowVec<CPL>::V*)(x + id) = v;
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < CPL; q++)
-                                if (th.okx[q]) x[id + q] = xn[q];
-                        }
-                    }
-                }
-            }
             issue(PB, P + 2);
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
@@ -211,27 +179,12 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     block_store<1>(acc, partials);
 }
 
-template <int CPL, int NW, int RW, bool XL>
+template <int CPL, int NW, int RW>
 __global__ __launch_bounds__(NW * 64) void k_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                                       float* __restrict__ pnew, float* __restrict__ Ap,
                                                       const uint8_t* __restrict__ nbr, Geo g, PmvTab K,
-                                                      RowGeo rg, double* partials, const Ctrl* ctrl, float* __restrict__ x) {
-    d_rows_pmv<CPL, NW, RW, XL>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x), x);
-}
-
-// x += alpha p on fluid cells (the x update still pending when the solve
-// stops), then p_out += x (cold start) or p_out = x (warm start), with
-// k_update's and k_finalize's roundings.
-template <typename T>
-__global__ void k_finalize_lag(const T* __restrict__ x, const T* __restrict__ p, const Ctrl* ctrl,
-                               double* __restrict__ pout, const uint8_t* __restrict__ nbr, long N, int assign) {
-    const double alpha = ctrl->alpha;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x, s = (long)gridDim.x * blockDim.x;
-    for (; t < N; t += s)
-        if (nbr[t] & NB_FLUID) {
-            const double xv = (double)(T)((double)x[t] + alpha * (double)p[t]);
-            pout[t] = assign ? xv : pout[t] + xv;
-        }
+                                                      RowGeo rg, double* partials, const Ctrl* ctrl) {
+    d_rows_pmv<CPL, NW, RW>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x));
 }
 
 }  // namespace uaamg
