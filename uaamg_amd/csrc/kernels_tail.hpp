@@ -10,7 +10,8 @@
 // records of the cells a thread owns in registers.  Arithmetic and order are
 // those of the per-pass kernels (k_smooth, k_restrict, k_prolong), i.e. of
 // the reference smooth / restrictResidual / prolongateAdd
-// (uaamg_preconditioner_3d.cpp:123-196).
+// (uaamg_preconditioner_3d.cpp:123-196); the FP32 form divides by the
+// hardware reciprocal, as the march kernels do.
 //
 // A thread owns pairs of x-adjacent cells (i odd, i+1): one cell of each
 // colour, so every RBGS pass keeps all lanes busy.
@@ -69,6 +70,15 @@ __device__ __forceinline__ void tail_load_pair(TailPair<CT>& P, const C4* __rest
     }
 }
 
+// (b + nb) / d: exact division in FP64 (the parity path); in FP32 the
+// hardware reciprocal, as in the march kernels of the same levels, which
+// shortens the dependent chain of every pass
+template <typename CT>
+__device__ __forceinline__ CT tdiv(CT num, CT d) {
+    if constexpr (std::is_same<CT, double>::value) return num / d;
+    else return num * __builtin_amdgcn_rcpf(d);
+}
+
 // One RBGS colour on the owned pairs of one level.
 template <int D, typename CT, int NP>
 __device__ __forceinline__ void tail_smooth(const TailPair<CT> (&P)[NP], CT* X, const CT* Bv, const Geo& g, int colour) {
@@ -88,7 +98,7 @@ __device__ __forceinline__ void tail_smooth(const TailPair<CT> (&P)[NP], CT* X, 
             const CT zp = q ? p.zp[1] : p.zp[0], zm = q ? p.zm[1] : p.zm[0];
             nb = nb + zp * X[id + g.sy] + zm * X[id - g.sy];
         }
-        X[id] = (Bv[id] + nb) / d;
+        X[id] = tdiv<CT>(Bv[id] + nb, d);
     }
 }
 
