@@ -150,7 +150,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     if (th.tile_row[t]) acc[0] += (double)pap;
                     if (th.tile_row[t]) {
                         const int id = th.roff[t] + R * (int)g.sy;
-                        if (th.okx[CPL - 1]) {
+                        if (th.full || th.okx[CPL - 1]) {
                             typename RowVec<CPL>::V vp, va;
 #pragma unroll
                             for (int q = 0; q < CPL; q++) { vp[q] = PN[S1][t][q]; va[q] = out[q]; }

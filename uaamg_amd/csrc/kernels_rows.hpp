@@ -158,9 +158,13 @@ struct RowThread {
     int roff[RW];
     bool okx[CPL];
     bool tile_row[RW];
+    bool full;        // every lane of every row is inside the grid (nx == 64 CPL)
     __device__ void init(const Geo& g, const RowGeo& rg, int ty, int kc) {
         lane = threadIdx.x & 63;
-        w = threadIdx.x >> 6;
+        // the wave index is uniform: in a scalar register, the row tests
+        // below (jrow, tile_row) are scalar branches, not exec-mask ones
+        w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        full = g.nx == 64 * CPL;
         i0 = 1 + CPL * lane;
         b0 = 1 + ty * TY;
         m0 = 1 + kc * rg.KC;
@@ -275,7 +279,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
                 if (pin && th.tile_row[t]) rr[0] += (double)rs;
                 if (pin && th.tile_row[t]) {
                     const int id = th.roff[t] + P * (int)g.sy;
-                    if (th.okx[CPL - 1]) {
+                    if (th.full || th.okx[CPL - 1]) {
                         typename RowVec<CPL>::V v;
 #pragma unroll
                         for (int q = 0; q < CPL; q++) v[q] = pfr[PB][t][q];
@@ -544,7 +548,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
                         red[1] += (double)s1;
                         red[2] += (double)s2;
                         const int id = th.roff[t] + R * (int)g.sy;
-                        if (th.okx[CPL - 1]) {
+                        if (th.full || th.okx[CPL - 1]) {
                             typename RowVec<CPL>::V v;
 #pragma unroll
                             for (int q = 0; q < CPL; q++) v[q] = X[S2][t][q];
@@ -926,7 +930,7 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
             for (int t = 0; t < RW; t++) {
                 if (!th.tile_row[t]) continue;
                 const int id = th.roff[t] + R * (int)g.sy;
-                if (std::is_same<SV, float>::value && th.okx[CPL - 1]) {
+                if (std::is_same<SV, float>::value && (th.full || th.okx[CPL - 1])) {
                     typename RowVec<CPL>::V v;
 #pragma unroll
                     for (int q = 0; q < CPL; q++) v[q] = Q.X[S2][t][q];
