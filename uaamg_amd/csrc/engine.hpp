@@ -502,10 +502,11 @@ public:
     }
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            if (fr_old_) {   // r = r_old - alpha Ap written to r; r.r -> convergence
+            if (fr_old_) {   // alpha from the operator's partials; r = r_old - alpha Ap written to r; r.r -> convergence
                 tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S> : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
-                        rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const Ctrl*)ctrl_);
+                        rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const double*)partials_m_,
+                        (int)pmv_blocks_, ctrl_);
                 k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
                 return;
             }
@@ -759,18 +760,21 @@ public:
         if (pmv_on_) {
             const T* pold = pbuf(pcur_);
             T* pnew = pbuf(pcur_ ^ 1);
+            // p.Ap partials in their own buffer: the down kernel that sums
+            // them writes its r.r partials while other workgroups still read
             if constexpr (D == 3 && std::is_same<T, float>::value) {
                 tlaunch(timer_, UAAMG_K_MATVEC, rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW> : k_rows_pmv<2, PMV_NW, PMV_RW>,
                         dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_, (const float*)pold, (float*)pnew,
-                        (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_, partials_, (const Ctrl*)ctrl_);
+                        (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_, partials_m_, (const Ctrl*)ctrl_);
             }
             pv = pnew;
             mb = pmv_blocks_;
+            if (!fr_) k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_m_, (int)mb, ctrl_);
         } else {
             tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
                     (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+            k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)mb, ctrl_);
         }
-        k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)mb, ctrl_);
         if (fr_) {   // r update and convergence test inside the level-0 down kernel
             fr_old_ = rbuf(pcur_);
             xfork_x_ = x;
@@ -923,6 +927,7 @@ private:
     // x += alpha p runs on a second stream.
     T* p2_ = nullptr;
     T* r2_ = nullptr;
+    double* partials_m_ = nullptr;       // p.Ap partials of k_rows_pmv
     bool pmv_on_ = false;
     bool fr_ = false;                    // this solve uses the split update
     const T* fr_old_ = nullptr;          // r_old of the fused down kernel (nullptr: plain)

@@ -190,15 +190,21 @@ struct RowThread {
 // writes the coarse cells whose 8 children lie in its tile.
 //
 // With FR (fused CG residual update, pcg_3d.cpp:225-231 for r) the kernel
-// reads r_old and Ap and works on r = r_old - alpha Ap (k_update's formula,
-// recomputed on the halo), writes that r on its tile to r_new (another
-// buffer: neighbouring workgroups still read r_old) and stores the partial
-// sums of r.r over its tile's fluid cells.
+// reads r_old and Ap and works on r = r_old - alpha Ap (recomputed on the
+// halo), writes that r on its tile to r_new (another buffer: neighbouring
+// workgroups still read r_old) and stores the partial sums of r.r over its
+// tile's fluid cells.  It also takes over the operator's finish
+// (k_fin_matvec): every workgroup sums the operator's p.Ap partials in the
+// same order and forms alpha = rsold / p.Ap itself; workgroup 0 records
+// p.Ap, alpha, the x update's alpha and a breakdown in the scalars, which no
+// other workgroup of this launch reads.
 struct RUpd {
     const float* Ap = nullptr;
     float* rnew = nullptr;
-    double* partials = nullptr;
-    const Ctrl* ctrl = nullptr;
+    double* partials = nullptr;      // r.r partials (this kernel's)
+    const double* pap = nullptr;     // p.Ap partials of the operator pass
+    int npap = 0;
+    Ctrl* ctrl = nullptr;
 };
 
 template <int CPL, int NW, int RW, typename VC, bool FR = false>
@@ -206,14 +212,37 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
                                              const RowTab& K, const RowGeo& rg, VC* __restrict__ bc, const Geo& cg,
                                              const int* skip, int tile, const RUpd& u = RUpd{}) {
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
-    if (skip && *skip) return;
+    if (skip && *skip) {
+        // a stopped solve: no x update for this iteration
+        if constexpr (FR) { if (blockIdx.x == 0 && threadIdx.x == 0) u.ctrl->xalpha = 0.0; }
+        return;
+    }
     __shared__ RowLDS<CPL, NW, RW> L;
+    float alpha = 0.0f;   // r update in FP32 (one FMA), as in k_rows_pmv
+    if constexpr (FR) {
+        __shared__ double s_pap;
+        double tot[1];
+        reduce_partials_at<1>(u.pap, u.npap, 0, u.npap, tot);
+        if (threadIdx.x == 0) s_pap = tot[0];
+        __syncthreads();
+        const double pAp = s_pap;
+        const bool ok = isfinite(pAp) && pAp >= 1e-15;   // pcg.cpp:100 breakdown test
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            Ctrl* c = u.ctrl;
+            c->pAp = pAp;
+            if (!isfinite(pAp)) { c->done = 1; c->status = ST_NONFINITE; c->iter += 1; c->alpha = 0.0; }
+            else if (pAp < 1e-15) { c->done = 1; c->status = ST_BREAKDOWN; c->iter += 1; c->alpha = 0.0; }
+            else c->alpha = c->rsold / pAp;
+            c->xalpha = c->alpha;
+        }
+        if (!ok) return;   // the whole solve stops; the finish after this kernel sees done
+        alpha = (float)(u.ctrl->rsold / pAp);
+    }
     if (tile >= rg.ntiles) {   // the whole workgroup leaves together
         if constexpr (FR) { double z0[1] = {0.0}; block_store<1>(z0, u.partials); }
         return;
     }
     const int ty = tile % rg.nty, kc = tile / rg.nty;
-    const float alpha = FR ? (float)u.ctrl->alpha : 0.0f;   // r update in FP32 (one FMA), as in k_rows_pmv
     double rr[1] = {0.0};
     RowThread<CPL, NW, RW> th;
     th.init(g, rg, ty, kc);
@@ -398,14 +427,16 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
     d_rows_down0<CPL, NW, RW, VC>(r, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x));
 }
 
-// down0 with the fused residual update (r_old -> r_new, r.r partials)
+// down0 with the fused residual update (r_old -> r_new, r.r partials) and
+// the operator's finish (alpha from the p.Ap partials)
 template <int CPL, int NW, int RW, typename VC>
 __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict__ rold, const uint8_t* __restrict__ nbr,
                                                          Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
                                                          const int* skip, const float* __restrict__ Ap,
-                                                         float* __restrict__ rnew, double* partials, const Ctrl* ctrl) {
+                                                         float* __restrict__ rnew, double* partials,
+                                                         const double* pap, int npap, Ctrl* ctrl) {
     RUpd u;
-    u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.ctrl = ctrl;
+    u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.pap = pap; u.npap = npap; u.ctrl = ctrl;
     d_rows_down0<CPL, NW, RW, VC, true>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
 }
 

@@ -53,7 +53,7 @@ void Engine<D, T, S>::allocate() {
     HIPCHK(hipHostMalloc((void**)&ctrl_host_, sizeof(Ctrl), hipHostMallocDefault));
     r_ = dalloc_margin<T>(N4_); z_ = dalloc_margin<T>(N4_); p_ = dalloc_margin<T>(N4_); Ap_ = dalloc_margin<T>(N4_);
     if (!x_is_p()) x_ = dalloc_margin<T>(N4_);
-    if (pmv_on_) { p2_ = dalloc_margin<T>(N4_); r2_ = dalloc_margin<T>(N4_); }
+    if (pmv_on_) { p2_ = dalloc_margin<T>(N4_); r2_ = dalloc_margin<T>(N4_); partials_m_ = dalloc<double>(pmv_blocks_ + 64); }
     long maxb = std::max<long>(lc0_.nblocks(), 2048);
     for (int l = 1; l < nl; l++) {
         Level& L = lv_[l];
@@ -549,6 +549,7 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     // fused CG kernels (3D FP32 forms with rows of at most 256 cells)
     const bool pmv = g->dim == 3 && o->precision != UAAMG_FP64 && g0.nx <= 256;
     size_t bytes = N0 + N4 + kRowSlack + 8 + sizeof(uaamg::Ctrl) + (pmv ? 7 : 5) * (N4 + 2 * margin) * sv;
+    if (pmv) bytes += (size_t)(8 * ((((g0.ny + 15) / 16) * ((g0.nz + 3) / 4) + 7) / 8) + 64) * 8;   // p.Ap partials
     long maxb = std::max<long>(cells_launch(g0, g->dim).nblocks(), 2048);
     for (size_t l = 1; l < shape.size(); l++) {
         bytes += (size_t)shape[l].N * (7 * sl + 1) + (size_t)kRowSlack * 6 * sl;
