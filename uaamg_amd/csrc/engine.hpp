@@ -518,7 +518,8 @@ public:
         if constexpr (D == 3 && std::is_same<T, float>::value) {
             tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? k_rows_up0<4, ROWS_NW, ROWS_RW, S> : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
-            k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+            // with the split update the next operator pass does this finish
+            if (!fr_) k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
         }
     }
 
@@ -762,10 +763,13 @@ public:
             T* pnew = pbuf(pcur_ ^ 1);
             // p.Ap partials in their own buffer: the down kernel that sums
             // them writes its r.r partials while other workgroups still read
+            // (with the split update it also finishes the level-0 up kernel's z sums)
             if constexpr (D == 3 && std::is_same<T, float>::value) {
-                tlaunch(timer_, UAAMG_K_MATVEC, rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW> : k_rows_pmv<2, PMV_NW, PMV_RW>,
-                        dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_, (const float*)pold, (float*)pnew,
-                        (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_, partials_m_, (const Ctrl*)ctrl_);
+                auto kern = fr_ ? (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, true> : k_rows_pmv<2, PMV_NW, PMV_RW, true>)
+                                : (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, false> : k_rows_pmv<2, PMV_NW, PMV_RW, false>);
+                tlaunch(timer_, UAAMG_K_MATVEC, kern, dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_,
+                        (const float*)pold, (float*)pnew, (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_,
+                        partials_m_, ctrl_, (const double*)partials_, (int)rows_blocks_);
             }
             pv = pnew;
             mb = pmv_blocks_;
@@ -932,17 +936,36 @@ private:
     bool fr_ = false;                    // this solve uses the split update
     const T* fr_old_ = nullptr;          // r_old of the fused down kernel (nullptr: plain)
     // The x update (x += alpha p, k_xupdate) runs on a second stream, forked
-    // after the level-0 down kernel so it overlaps the latency-bound coarse
-    // levels, and joined before the next operator pass (or the end of a
-    // captured chunk / of the solve).  Default priority, like stream_.
+    // inside the V-cycle so it overlaps the latency-bound coarse levels, and
+    // joined before the next operator pass (or the end of a captured chunk /
+    // of the solve).  Default priority, like stream_.
     hipStream_t xs_ = nullptr;
-    // fork point: after the down kernel of this level, or (default, any value
-    // >= the tail level) right before the tail: the one-workgroup tail leaves
-    // the chip's bandwidth to the update (30.8 vs 29.6 solves/s at 256^3 when
-    // forked after the level-0 down kernel, where it slows the coarse levels)
+    // Fork point: after the down kernel of this level, or (any value >= the
+    // tail level) right before the tail.  The update runs as one 256-thread
+    // workgroup per CU: it then takes longer but leaves the coarse kernels
+    // room.  Measured at 256^3 (solves/s): 2048 workgroups forked before the
+    // tail 31.5; 256 workgroups forked after level 2 32.4; 192 or 128
+    // workgroups 30.3; 256 x 512 or 1024 threads 31.8 / 30.4; forked after
+    // level 0 with 128 workgroups 26.7.  UAAMG_XFORK / UAAMG_XBLOCKS /
+    // UAAMG_XTHREADS override.
     int xfork_level_ = [] {
         const char* e = std::getenv("UAAMG_XFORK");
-        return e ? std::max(0, std::atoi(e)) : 99;
+        return e ? std::max(0, std::atoi(e)) : 2;
+    }();
+    unsigned xblocks_ = [] {
+        const char* e = std::getenv("UAAMG_XBLOCKS");
+        return e ? (unsigned)std::max(8, std::atoi(e)) : 256u;
+    }();
+    unsigned xthreads_ = [] {
+        const char* e = std::getenv("UAAMG_XTHREADS");
+        return e ? (unsigned)std::min(1024, std::max(64, std::atoi(e) / 64 * 64)) : 256u;
+    }();
+    // whether the update reads the mask (UAAMG_XNB=1); by default it does
+    // not: the operator pass writes p = 0 on every non-fluid cell and p's
+    // ghosts stay 0, so x + alpha p leaves those cells of x as they are
+    bool xnb_ = [] {
+        const char* e = std::getenv("UAAMG_XNB");
+        return e ? e[0] != '0' : false;
     }();
     hipEvent_t xfork_ = nullptr, xjoin_ = nullptr;
     bool xpend_ = false;
@@ -951,8 +974,10 @@ private:
     void fork_x() {
         HIPCHK(hipEventRecord(xfork_, stream_));
         HIPCHK(hipStreamWaitEvent(xs_, xfork_, 0));
-        tlaunch(timer_, UAAMG_K_UPDATE, k_xupdate<T>, dim3(flat_blocks(2048)), dim3(256), xs_,
-                xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const Ctrl*)ctrl_);
+        const long xneed = (N4_ / 4 + xthreads_ - 1) / xthreads_;
+        const unsigned xb = (unsigned)std::max<long>(1, std::min<long>(xneed, xblocks_));
+        tlaunch(timer_, UAAMG_K_UPDATE, xnb_ ? k_xupdate<T, true> : k_xupdate<T, false>, dim3(xb),
+                dim3(xthreads_), xs_, xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const Ctrl*)ctrl_);
         HIPCHK(hipEventRecord(xjoin_, xs_));
         xpend_ = true;
     }

@@ -443,8 +443,10 @@ __global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
 
 // x += alpha p on fluid cells, alpha = ctrl->xalpha (the x half of k_update,
 // launched on its own; it does not read the done flag, which the r half may
-// set while it runs).
-template <typename T>
+// set while it runs).  With NB = false the mask is not read: for a p that is
+// exactly 0 on every non-fluid cell and ghost (the operator pass's p), the
+// update leaves those cells of x unchanged by itself.
+template <typename T, bool NB = true>
 __global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint8_t* __restrict__ nbr, long N4,
                           const Ctrl* ctrl) {
     const double alpha = ctrl->xalpha;
@@ -452,12 +454,15 @@ __global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint
     const int fv_b = blockIdx.x, fv_n = gridDim.x;
     FLAT_LOOP(N4) {
         long c = 4 * v_;
-        uint32_t w = *(const uint32_t*)(nbr + c);
-        if (!(w & 0x01010101u)) continue;
+        uint32_t w = 0x01010101u;
+        if constexpr (NB) {
+            w = *(const uint32_t*)(nbr + c);
+            if (!(w & 0x01010101u)) continue;
+        }
         vec4<T> xv = *(const vec4<T>*)(x + c), pv = *(const vec4<T>*)(p + c);
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            if (nbyte(w, q) & NB_FLUID) xv[q] = (T)((double)xv[q] + alpha * (double)pv[q]);
+            if (!NB || (nbyte(w, q) & NB_FLUID)) xv[q] = (T)((double)xv[q] + alpha * (double)pv[q]);
         *(vec4<T>*)(x + c) = xv;
     }
 }

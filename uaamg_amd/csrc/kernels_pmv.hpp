@@ -45,51 +45,88 @@ struct PmvLDS {
     float ex[2][NW][2][CPL][64];   // [step parity][wave][first / last row][cell][lane]
 };
 
-template <int CPL, int NW, int RW>
+// With FZ the pass also does the finish of the z sums (finish_zsums,
+// pcg_3d.cpp:229-241): every workgroup adds the level-0 up kernel's partials
+// of sum z, sum r.z, sum r in the same order and forms the same mean_z, r.z
+// and beta; workgroup 0 stores them.  rsold and the first-iteration flag are
+// read here and only written by the next kernel (the level-0 down kernel),
+// so no workgroup can see a value of this launch.
+template <int CPL, int NW, int RW, bool FZ = false>
 __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                            float* __restrict__ pnew, float* __restrict__ Ap,
                                            const uint8_t* __restrict__ nbr, const Geo& g, const PmvTab& K,
-                                           const RowGeo& rg, double* partials, const Ctrl* ctrl, int tile) {
+                                           const RowGeo& rg, double* partials, Ctrl* ctrl, int tile,
+                                           const double* zp = nullptr, int nzp = 0) {
     if (ctrl->done) return;
     __shared__ PmvLDS<CPL, NW> L;
     __shared__ double sd[64];
-    for (int e = threadIdx.x; e < 64; e += NW * 64) sd[e] = K.d[e];
-    __syncthreads();
+    __shared__ double s_zs[3];
     double acc[1] = {0.0};
     const bool active = tile < rg.ntiles;   // idle workgroups still store a zero partial
     const int ty = tile % rg.nty, kc = tile / rg.nty;
     RowThread<CPL, NW, RW, 1> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
+
+    float PN[4][RW][CPL];        // p_new of planes (slot = plane & 3)
+    uint8_t NB[4][RW][CPL];
+    float pfz[2][RW][CPL], pfp[2][RW][CPL];
+    uint8_t pfn[2][RW][CPL];
+    auto issue = [&](int slot, int P) {
+        const int po = th.plane(g, P);
+#pragma unroll
+        for (int t = 0; t < RW; t++) {
+            row_load<CPL>(z, nbr, th.roff[t] + po, th.okx, pfz[slot][t], pfn[slot][t]);
+            const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
+#pragma unroll
+            for (int q = 0; q < CPL; q++) pfp[slot][t][q] = v[q];
+        }
+    };
+    // the z-sum partials first, then the first two planes: both sets of
+    // loads are in flight while the partials are added
+    double zs[3] = {0.0, 0.0, 0.0};
+    if constexpr (FZ) {
+        for (int x = lin_thread(); x < nzp; x += n_threads())
+#pragma unroll
+            for (int q = 0; q < 3; q++) zs[q] += zp[(long)q * nzp + x];
+    }
+    if (active) {
+        issue(0, th.m0 - 1);
+        issue(1, th.m0);
+    }
+    for (int e = threadIdx.x; e < 64; e += NW * 64) sd[e] = K.d[e];
+    if constexpr (FZ) {
+        block_sum<3>(zs);
+        if (threadIdx.x == 0) { s_zs[0] = zs[0]; s_zs[1] = zs[1]; s_zs[2] = zs[2]; }
+    }
+    __syncthreads();
     // the p update runs in FP32 (one FMA) with the FP64 scalars rounded once:
     // it differs from k_pupdate's FP64 form by the rounding of the scalars,
     // far below the FP32 storage of the vectors
-    const float mean = (float)ctrl->mean_z, beta = (float)ctrl->beta;
+    float mean, beta;
+    if constexpr (FZ) {
+        const double sz = s_zs[0], srz = s_zs[1], sr = s_zs[2];
+        const double m = (ctrl->project && ctrl->count > 0) ? sz / ctrl->count : 0.0;
+        const double rz = srz - m * sr;
+        const double b = ctrl->first ? 0.0 : rz / ctrl->rsold;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->sum_z = sz; ctrl->sum_rz = srz; ctrl->sum_r = sr;
+            ctrl->mean_z = m; ctrl->rz = rz; ctrl->beta = b;
+            if (!isfinite(rz)) { ctrl->done = 1; ctrl->status = ST_NONFINITE; }
+        }
+        if (!isfinite(rz)) return;   // the whole solve stops; the down kernel sees done
+        mean = (float)m; beta = (float)b;
+    } else {
+        mean = (float)ctrl->mean_z; beta = (float)ctrl->beta;
+    }
 
     if (active) {
-        float PN[4][RW][CPL];        // p_new of planes (slot = plane & 3)
-        uint8_t NB[4][RW][CPL];
-        float pfz[2][RW][CPL], pfp[2][RW][CPL];
-        uint8_t pfn[2][RW][CPL];
 #pragma unroll
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int t = 0; t < RW; t++)
 #pragma unroll
                 for (int q = 0; q < CPL; q++) { PN[s][t][q] = 0.0f; NB[s][t][q] = 0; }
-
-        auto issue = [&](int slot, int P) {
-            const int po = th.plane(g, P);
-#pragma unroll
-            for (int t = 0; t < RW; t++) {
-                row_load<CPL>(z, nbr, th.roff[t] + po, th.okx, pfz[slot][t], pfn[slot][t]);
-                const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
-#pragma unroll
-                for (int q = 0; q < CPL; q++) pfp[slot][t][q] = v[q];
-            }
-        };
-        issue(0, th.m0 - 1);
-        issue(1, th.m0);
 
         // step: plane P enters (slot S); the operator on plane P-1
         auto step = [&](auto Sc, int P) {
@@ -179,12 +216,14 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     block_store<1>(acc, partials);
 }
 
-template <int CPL, int NW, int RW>
+template <int CPL, int NW, int RW, bool FZ = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                                       float* __restrict__ pnew, float* __restrict__ Ap,
                                                       const uint8_t* __restrict__ nbr, Geo g, PmvTab K,
-                                                      RowGeo rg, double* partials, const Ctrl* ctrl) {
-    d_rows_pmv<CPL, NW, RW>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x));
+                                                      RowGeo rg, double* partials, Ctrl* ctrl,
+                                                      const double* zp, int nzp) {
+    d_rows_pmv<CPL, NW, RW, FZ>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x),
+                                zp, nzp);
 }
 
 }  // namespace uaamg

@@ -195,9 +195,10 @@ struct RowThread {
 // workgroups still read r_old) and stores the partial sums of r.r over its
 // tile's fluid cells.  It also takes over the operator's finish
 // (k_fin_matvec): every workgroup sums the operator's p.Ap partials in the
-// same order and forms alpha = rsold / p.Ap itself; workgroup 0 records
-// p.Ap, alpha, the x update's alpha and a breakdown in the scalars, which no
-// other workgroup of this launch reads.
+// same order and forms alpha = (r, z) / p.Ap itself (pcg_3d.cpp:222; (r, z)
+// as the z-sums finish stored it); workgroup 0 records p.Ap, alpha, the x
+// update's alpha, a breakdown, and rsold = (r, z) for the next beta, in
+// scalars that no other workgroup of this launch reads.
 struct RUpd {
     const float* Ap = nullptr;
     float* rnew = nullptr;
@@ -232,11 +233,15 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
             c->pAp = pAp;
             if (!isfinite(pAp)) { c->done = 1; c->status = ST_NONFINITE; c->iter += 1; c->alpha = 0.0; }
             else if (pAp < 1e-15) { c->done = 1; c->status = ST_BREAKDOWN; c->iter += 1; c->alpha = 0.0; }
-            else c->alpha = c->rsold / pAp;
+            else c->alpha = c->rz / pAp;
             c->xalpha = c->alpha;
+            // the operator pass read rsold and the first flag; nothing in
+            // this launch does
+            c->rsold = c->rz;
+            c->first = 0;
         }
         if (!ok) return;   // the whole solve stops; the finish after this kernel sees done
-        alpha = (float)(u.ctrl->rsold / pAp);
+        alpha = (float)(u.ctrl->rz / pAp);
     }
     if (tile >= rg.ntiles) {   // the whole workgroup leaves together
         if constexpr (FR) { double z0[1] = {0.0}; block_store<1>(z0, u.partials); }
