@@ -53,11 +53,17 @@ def test_fused_matches_separate_passes(gpu, name, pb, use_graph):
     assert np.abs(a - b).max() <= 2e-6 * np.abs(b).max()
 
 
+# a grid whose level 1 runs on the record row kernels (64 < nx/2 <= 128), as
+# at 256^3: the stop paths with those kernels in the V-cycle
+L1REC = [k for k, c in enumerate(FUSED) if c[0] == "3d_200x70x40_walls_solids"][0]
+
+
+@pytest.mark.parametrize("case", [0, L1REC])
 @pytest.mark.parametrize("max_iter", [1, 5, 6])
-def test_fused_max_iter_stop(gpu, max_iter):
+def test_fused_max_iter_stop(gpu, max_iter, case):
     """Stopped by max_iter (inside and at the end of a 4-iteration graph
     chunk): the x of the last iteration is in p, as in the separate passes."""
-    name, pb = FUSED[0]
+    name, pb = FUSED[case]
     rhs = cases.gaussian_rhs(pb)
     p1, s1 = _solve(pb, True, rhs, max_iter, 1e-12, use_graph=1, check_every=4)
     p0, s0 = _solve(pb, False, rhs, max_iter, 1e-12, use_graph=1, check_every=4)
@@ -67,10 +73,11 @@ def test_fused_max_iter_stop(gpu, max_iter):
     assert np.abs(p1[m] - p0[m]).max() <= 1e-5 * np.abs(p0[m]).max()
 
 
-def test_fused_zero_rhs(gpu):
+@pytest.mark.parametrize("case", [0, L1REC])
+def test_fused_zero_rhs(gpu, case):
     """A zero rhs stops at the first operator (p.Ap = 0 < 1e-15, pcg.cpp:100):
     the same status and iteration count as the separate passes, p unchanged."""
-    name, pb = FUSED[0]
+    name, pb = FUSED[case]
     rhs = np.zeros(pb.N)
     p0 = np.random.default_rng(3).standard_normal(pb.N)
     for pmv in (True, False):
@@ -85,13 +92,14 @@ def test_fused_zero_rhs(gpu):
             assert (st.status, st.iters) == ref
 
 
-def test_fused_warm_start_and_repeat(gpu):
+@pytest.mark.parametrize("case", [1, L1REC])
+def test_fused_warm_start_and_repeat(gpu, case):
     """Two solves in a row on one context (buffers alternate between solves)
     agree bit for bit, and a warm start from the converged pressure (relres
     measured against its own, already small, initial residual) stays at that
     pressure."""
     import torch
-    name, pb = FUSED[1]
+    name, pb = FUSED[case]
     rhs = cases.gaussian_rhs(pb)
     c = _ctx(pb, True, precision=U.FP32, tol_mode=U.TOL_REL, use_graph=1, check_every=4)
     dev = torch.device("cuda:0")
