@@ -118,3 +118,29 @@ def test_fused_warm_start_and_repeat(gpu, case):
     m = pb.fluid_interior_mask()
     a, b = pa.cpu().numpy()[m], first.cpu().numpy()[m]
     assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("env", [{"UAAMG_XNB": "1"}, {"UAAMG_XFORK": "0", "UAAMG_XBLOCKS": "2048"},
+                                 {"UAAMG_XFORK": "99", "UAAMG_XTHREADS": "512"}],
+                         ids=["mask", "fork0_2048", "fork_tail_512"])
+def test_fused_x_update_forms(gpu, env):
+    """The side-stream x update (x += alpha p, elementwise) in its other forms
+    (mask read, other fork points and launch shapes) gives bit-identical
+    pressure to the default form: only where and how the same FP64 updates
+    run changes."""
+    name, pb = FUSED[L1REC]
+    rhs = cases.gaussian_rhs(pb)
+    p_def, s_def = _solve(pb, True, rhs, 500, 1e-6, use_graph=1, check_every=4)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        p_alt, s_alt = _solve(pb, True, rhs, 500, 1e-6, use_graph=1, check_every=4)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    assert s_def.kernel_set == s_alt.kernel_set == 2
+    assert s_def.status == s_alt.status == U.OK and s_def.iters == s_alt.iters
+    np.testing.assert_array_equal(p_alt, p_def)
