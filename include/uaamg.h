@@ -3,7 +3,7 @@
  * solve.  Plain pointers and sizes only; every entry point returns a status
  * code (UAAMG_OK == 0).  Thread safety: one context per host thread.
  *
- * Layout contract (reference include/core/mesh.h:32-35, mesh_3d.h:78-82):
+ * Layout contract (reference include/core/mesh.h:32-35, mesh_3d.h:32-38):
  * cell arrays are ghost-padded, i fastest,
  *     idx = i + j*(nx+2) [+ k*(nx+2)*(ny+2)],   interior i,j,k = 1..n,
  * spacing h = L/n (mesh.h:21-22).  For 2D pass dim = 2 and n[2] = 1.
@@ -64,7 +64,7 @@ typedef struct {
     int check_every;           /* CG iterations queued between host convergence checks (>=1) */
     int use_graph;             /* 1: replay queued iterations from a captured hipGraph */
     int timing;                /* bitmask of UAAMG_K_* kernel classes timed with HIP events (0 off, 0xff all) */
-    int flexible;              /* 0: beta = (r,z)/rsold (pcg_3d.cpp:241); 1: Polak-Ribiere (flexible CG) */
+    int flexible;              /* 0: beta = (r,z)/rsold (pcg_3d.cpp:115); 1: Polak-Ribiere (flexible CG) */
 } uaamg_opts;
 
 typedef struct {
@@ -94,7 +94,7 @@ typedef struct uaamg_ctx uaamg_ctx;
 int uaamg_default_opts(uaamg_opts* o);
 
 /* Replaces constructing PCG(UAAMGPreconditioner) / CudaPCG3D
- * (src/solver/factory.cpp:23, include/solver/cuda/cuda_pcg_3d.h:39-81):
+ * (src/solver/factory.cpp:23, include/solver/cuda/cuda_pcg_3d.h:7-49):
  * one context owns the device hierarchy and CG buffers. */
 int uaamg_create(uaamg_ctx** out, const uaamg_opts* o);
 void uaamg_destroy(uaamg_ctx* ctx);
@@ -130,7 +130,7 @@ int uaamg_solve_device(uaamg_ctx* ctx, const double* rhs_dev, double* p_dev,
  * uaamg_preconditioner.cpp:177-199). Host FP64 padded arrays; interior of z written. */
 int uaamg_apply_precond(uaamg_ctx* ctx, const double* r_host, double* z_host);
 
-/* The PCG operator Av (pcg.cpp:35-61, pcg_3d.cpp:164-181) and p.Av over fluid
+/* The PCG operator Av (pcg.cpp:35-61, pcg_3d.cpp:38-55) and p.Av over fluid
  * cells, as the HIP matvec+dot kernel computes them. Host FP64 arrays. */
 int uaamg_matvec(uaamg_ctx* ctx, const double* v_host, double* Av_host, double* vAv);
 

@@ -5,7 +5,7 @@
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product (uaamg_amd/, include/uaamg.h) never calls it.
  *
- * Layout contract (include/core/mesh.h:32-35, mesh_3d.h:78-82): cell arrays
+ * Layout contract (include/core/mesh.h:32-35, mesh_3d.h:32-38): cell arrays
  * are ghost-padded, i fastest: idx = i + j*(nx+2) [+ k*(nx+2)*(ny+2)],
  * interior indices 1..n.  For dim == 2 pass nz = 1 and use k = 0.
  *
@@ -68,7 +68,7 @@ int orc_level_data(const orc_grid* g, const orc_opts* o, int level,
                    unsigned char* solid, double* diag, double* cx, double* cy,
                    double* cz, double* excess);
 
-/* PCG matvec with uniform constants (pcg.cpp:47-59, pcg_3d.cpp:164-181),
+/* PCG matvec with uniform constants (pcg.cpp:47-59, pcg_3d.cpp:38-55),
  * extended with ghost = -v on Dirichlet walls. Writes interior cells. */
 int orc_matvec(const orc_grid* g, const double* v, double* Av);
 

@@ -122,7 +122,7 @@ def main():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array(kind), **d)
         index.append(name)
         print("wrote", name)
-    # G4: standalone amgpcg (test_amgpcg.cpp:172-189 inputs) and iteration counts
+    # G4: standalone amgpcg (test_amgpcg.cpp:77-85 inputs, 2 pi^2 sin sin) and iteration counts
     for N in (32, 64, 128, 256):
         b = np.zeros((N + 2) ** 2)
         h = 1.0 / (N + 1)

@@ -32,14 +32,14 @@ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k) {
 // bit0 fluid, bit1 +x, bit2 -x, bit3 +y, bit4 -y, bit5 +z, bit6 -z.
 // A coupling exists iff both cells are fluid and the face is interior
 // (setupFineCoeffs, uaamg_preconditioner_3d.cpp:58-66); for a fluid cell the
-// -x bit equals the PCG matvec's "(i>1 && !solid(i-1))" test (pcg_3d.cpp:172).
+// -x bit equals the PCG matvec's "(i>1 && !solid(i-1))" test (pcg_3d.cpp:46).
 enum : uint8_t { NB_FLUID = 1, NB_XP = 2, NB_XM = 4, NB_YP = 8, NB_YM = 16, NB_ZP = 32, NB_ZM = 64 };
 
 // Per-level uniform constants in compute type CT.
 template <typename CT>
 struct Consts {
     CT idx2, idy2, idz2;       // 1/h^2 per axis at this level (level 0 only: fine constants)
-    CT diag_u;                 // 2*(idx2+idy2[+idz2]) of the PCG matvec (pcg_3d.cpp:166)
+    CT diag_u;                 // 2*(idx2+idy2[+idz2]) of the PCG matvec (pcg_3d.cpp:40)
     CT ex2[6];                 // Dirichlet diagonal excess per face (2/h^2), 0 for Neumann
     int wall[6];               // 1 = Dirichlet
     int any_dirichlet;

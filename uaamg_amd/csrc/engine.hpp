@@ -599,7 +599,7 @@ public:
 
     // ---------------- solve ----------------
     // Initial residual, its projection, the first V-cycle and p: everything
-    // before the first CG iteration (pcg_3d.cpp:150-215), on the bound Ctrl.
+    // before the first CG iteration (pcg_3d.cpp:22-89), on the bound Ctrl.
     // With the fused p update (pmv_on_ and fused_p), the first p = z - mean is
     // formed by the first iteration's k_rows_pmv instead of k_pupdate.
     void start(const double* rhs_dev, double* p_dev, int warm, int max_iter, double tol, bool fused_p = false) {

@@ -387,7 +387,7 @@ __global__ void k_fin_update(const double* partials, int nb, Ctrl* ctrl) {
 }
 
 // Shared tail of the z reductions: mean of z, (r, z - mean), beta, rsold.
-// beta = (r_new, z_new)/rsold (pcg_3d.cpp:241), or with ctrl->flexible the
+// beta = (r_new, z_new)/rsold (pcg_3d.cpp:115), or with ctrl->flexible the
 // Polak-Ribiere form (r_new, z_new - z_old)/rsold, which tolerates a
 // preconditioner that is not exactly symmetric (FP32/BF16 levels).
 __device__ __forceinline__ void finish_zsums(Ctrl* ctrl, double sz, double srz, double sr) {

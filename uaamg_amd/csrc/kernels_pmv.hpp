@@ -2,7 +2,7 @@
 // (FP32 vectors, 3D).
 //
 // One pass replaces the p update of iteration k-1 and the operator of
-// iteration k (pcg_3d.cpp:233-236 and 219-223):
+// iteration k (pcg_3d.cpp:116-122 and 94-96):
 //   p_new = (z - mean_z) + beta p_old      on fluid cells (0 elsewhere)
 //   Ap    = d p_new - sum_nb c p_new_nb     (FP64 arithmetic on the FP32
 //                                            values, rounded once; d from the
@@ -46,7 +46,7 @@ struct PmvLDS {
 };
 
 // With FZ the pass also does the finish of the z sums (finish_zsums,
-// pcg_3d.cpp:229-241): every workgroup adds the level-0 up kernel's partials
+// pcg_3d.cpp:113-115): every workgroup adds the level-0 up kernel's partials
 // of sum z, sum r.z, sum r in the same order and forms the same mean_z, r.z
 // and beta; workgroup 0 stores them.  rsold and the first-iteration flag are
 // read here and only written by the next kernel (the level-0 down kernel),

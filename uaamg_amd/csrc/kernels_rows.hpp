@@ -189,13 +189,13 @@ struct RowThread {
 // r -> coarse b (restricted residual after the forward sweep).  A workgroup
 // writes the coarse cells whose 8 children lie in its tile.
 //
-// With FR (fused CG residual update, pcg_3d.cpp:225-231 for r) the kernel
+// With FR (fused CG residual update, pcg_3d.cpp:106 for r) the kernel
 // reads r_old and Ap and works on r = r_old - alpha Ap (recomputed on the
 // halo), writes that r on its tile to r_new (another buffer: neighbouring
 // workgroups still read r_old) and stores the partial sums of r.r over its
 // tile's fluid cells.  It also takes over the operator's finish
 // (k_fin_matvec): every workgroup sums the operator's p.Ap partials in the
-// same order and forms alpha = (r, z) / p.Ap itself (pcg_3d.cpp:222; (r, z)
+// same order and forms alpha = (r, z) / p.Ap itself (pcg_3d.cpp:99; (r, z)
 // as the z-sums finish stored it); workgroup 0 records p.Ap, alpha, the x
 // update's alpha, a breakdown, and rsold = (r, z) for the next beta, in
 // scalars that no other workgroup of this launch reads.
