@@ -499,7 +499,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the benchmark runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent solves shard per GPU with no communication; a single solve would need z-slab halos.", file=sys.stderr)
         sys.exit(2)
 

@@ -231,7 +231,7 @@ private:
     uaamg_stats last_{};
 };
 
-// Factory::create for this build: "pcg_uaamg" on the HIP backend is the only
+// Factory::create of this library: "pcg_uaamg" on the HIP backend is the only
 // key with a back-end here; every other key (or the CPU backend) throws
 // instead of the reference's silent fallback to Jacobi (factory.cpp:24),
 // so a misspelled key cannot quietly select a different solver.
@@ -240,7 +240,7 @@ struct Factory {
                                           const uaamg_opts* o = nullptr) {
         if (parse_solver_kind(key) == SolverKind::PCG_UAAMG && be == Backend::HIP)
             return std::make_unique<HipPCGSolver>(o);
-        throw std::invalid_argument("solver key '" + key + "' has no back-end in this build");
+        throw std::invalid_argument("solver key '" + key + "' has no back-end in this library");
     }
 };
 struct Factory3D {
@@ -248,7 +248,7 @@ struct Factory3D {
                                             const uaamg_opts* o = nullptr) {
         if (parse_solver_kind(key) == SolverKind::PCG_UAAMG && be == Backend::HIP)
             return std::make_unique<HipPCGSolver3D>(o);
-        throw std::invalid_argument("solver key '" + key + "' has no back-end in this build");
+        throw std::invalid_argument("solver key '" + key + "' has no back-end in this library");
     }
 };
 
@@ -258,14 +258,14 @@ struct Factory3D {
 struct PressureProjection {
     static void project(Grid& g, double dt, Solver& solver, int max_iter, double tol) {
         auto* s = dynamic_cast<HipPCGSolver*>(&solver);
-        if (!s) throw std::invalid_argument("PressureProjection::project needs HipPCGSolver in this build");
+        if (!s) throw std::invalid_argument("PressureProjection::project needs HipPCGSolver (the HIP solver)");
         s->project(g, dt, max_iter, tol);
     }
 };
 struct PressureProjection3D {
     static void project(Grid3D& g, double dt, Solver3D& solver, int max_iter, double tol) {
         auto* s = dynamic_cast<HipPCGSolver3D*>(&solver);
-        if (!s) throw std::invalid_argument("PressureProjection3D::project needs HipPCGSolver3D in this build");
+        if (!s) throw std::invalid_argument("PressureProjection3D::project needs HipPCGSolver3D (the HIP solver)");
         s->project(g, dt, max_iter, tol);
     }
 };
