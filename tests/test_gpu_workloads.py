@@ -94,3 +94,24 @@ def test_plume3d_fp32_vs_fp64(gpu, n):
     a = a - a.mean()
     b = b - b.mean()
     assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max()
+
+
+def test_plume3d_host_check_schedules_agree(gpu):
+    """On a grid of at least 2^21 cells the host queues as many chunks as the
+    residual's decay rate predicts before it checks; chunks queued past the
+    convergence only run early exits.  Every schedule (one iteration per
+    check, chunks of 4 scheduled by the decay rate, chunks of 64, eager or
+    graph replay) gives the same iteration count and bit-identical pressure."""
+    pb = W.plume3d(seed=0, n=128)
+    assert (pb.n[0] + 2) ** 3 >= 1 << 21
+    res = []
+    for every, graph in ((1, 0), (4, 1), (4, 0), (64, 1)):
+        c = U.Context(precision=U.FP32, tol_mode=U.TOL_REL, check_every=every, use_graph=graph)
+        c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+        p, st = c.solve(pb.rhs, 400, 1e-6)
+        c.close()
+        assert st.status == 0, (every, graph, U.status_name(st.status))
+        res.append((every, graph, st.iters, p))
+    for every, graph, it, p in res[1:]:
+        assert it == res[0][2], (every, graph, it, res[0][2])
+        np.testing.assert_array_equal(p, res[0][3])
