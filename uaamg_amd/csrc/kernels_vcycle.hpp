@@ -41,6 +41,30 @@ __device__ __forceinline__ Sten<CT> sten_l0(uint8_t nb, const Consts<CT>& c, con
     return s;
 }
 
+// BF16 records hold the diagonal rounded up to the next BF16 value (the
+// coarse operator stays diagonally dominant; k_level_diag).  On a cell with no
+// Dirichlet excess that is the rounded-up sum of its six stored couplings, a
+// small lift that makes a pure-Neumann level non-singular and puts the
+// residual a BF16 V-cycle leaves in its smoothest modes.  Where the six
+// couplings are at hand, a stored diagonal equal to their rounded-up sum is
+// replaced by the sum itself (exact in FP32: six BF16 values of close
+// magnitude), so such a level keeps its zero row sums; a diagonal with
+// Dirichlet excess differs from that sum by far more than one BF16 step and
+// is kept.  Other record types store the diagonal unrounded.
+template <typename C4, typename CT>
+__device__ __forceinline__ CT snap_diag(CT dstored, CT csum) {
+    if constexpr (std::is_same<C4, BF16x4>::value) {
+        if (!(dstored > CT(0))) return dstored;
+        const float cs = (float)csum;
+        __bf16 r = (__bf16)cs;
+        if ((float)r < cs) r = __builtin_bit_cast(__bf16, (uint16_t)(__builtin_bit_cast(uint16_t, r) + 1));
+        return ((float)r == (float)dstored) ? csum : dstored;
+    } else {
+        (void)csum;
+        return dstored;
+    }
+}
+
 // Levels >= 1: stored record {cx, cy, cz, diag} (diag < 0 marks a solid
 // cell); the -x/-y/-z couplings of a cell live at its neighbour.
 template <int D, typename CT, typename C4>
@@ -54,6 +78,7 @@ __device__ __forceinline__ Sten<CT> sten_lc(const C4* __restrict__ coef, const G
     s.ym = (CT)coef[id - g.sx].y;
     if (D == 3) { s.zp = (CT)c0.z; s.zm = (CT)coef[id - g.sy].z; }
     else { s.zp = CT(0); s.zm = CT(0); }
+    if (s.fluid) s.d = snap_diag<C4, CT>(s.d, s.xp + s.xm + s.yp + s.ym + s.zp + s.zm);
     return s;
 }
 
