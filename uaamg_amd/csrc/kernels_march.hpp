@@ -127,7 +127,7 @@ __device__ __forceinline__ MCell<L0, CT> mload_cell(const uint8_t* __restrict__ 
 // Stencil of cell (position u) at a plane: own cell state c, -a/-b couplings
 // from the LDS slot of that plane (Lc), -m coupling from the previous plane's
 // own state cprev (Lc).
-template <int D, bool L0, int RX, int RY, typename CT>
+template <int D, bool L0, int RX, int RY, typename CT, bool XD = false>
 __device__ __forceinline__ MSten<D, L0, CT> mst(const MCell<L0, CT>& c, const MCell<L0, CT>& cprev,
                                                 const MarchLDS<D, L0, RX, RY, CT>& S, int slot, int e,
                                                 const Consts<CT>& k0, const Geo& g, int i, int j, int k) {
@@ -145,6 +145,8 @@ __device__ __forceinline__ MSten<D, L0, CT> mst(const MCell<L0, CT>& c, const MC
         if (D == 3) { t.bp = c.cy; t.bm = S.cb[slot][e - RX]; }
         else { t.bp = CT(0); t.bm = CT(0); }
         t.mp = c.cz; t.mm = cprev.cz;
+        // BF16 records: the exact diagonal where it carries no excess (snap_diag)
+        if constexpr (XD) { if (t.fluid) t.d = snap_diag<BF16x4, CT>(c.d, t.ap + t.am + t.bp + t.bm + t.mp + t.mm); }
         return t;
     }
 }
@@ -156,6 +158,10 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
                                                 const int* skip, int bx, int by, int bz) {
     using CT = typename CTOf<VB>::type;
     using Cell = MCell<L0, CT>;
+    // BF16 records: the diagonal of a plane's cells is formed once its -x/-y
+    // couplings are in LDS (after the plane's barrier), so the odd cells'
+    // b/d of an entering plane moves behind that barrier
+    constexpr bool XD = !L0 && std::is_same<C4, BF16x4>::value;
     constexpr int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1, RN = RX * RY;
     static_assert(RX * RY / PPT == 256, "256 threads");
     if (skip && *skip) return;
@@ -191,6 +197,11 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
     for (int u = 0; u < PPT; u++) {
         for (int q = 0; q < 3; q++) qb[u][q] = CT(0);
         for (int q = 0; q < 4; q++) { qx[u][q] = CT(0); qc[u][q] = mload_cell<L0, CT, C4, D>(A.nbr, A.rec, 0, false); }
+        // the -z coupling of the first halo plane: plane m0 - 3's record
+        if constexpr (XD) {
+            const bool v3 = pos.in[u] && m0 - 3 >= 1 && m0 - 3 <= nm;
+            qc[u][3] = mload_cell<L0, CT, C4, D>(A.nbr, A.rec, pos.off[u] + (long)(m0 - 3) * ms, v3);
+        }
     }
     using RT = typename std::conditional<std::is_same<CT, double>::value, double, float>::type;
 
@@ -205,16 +216,31 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
             qc[u][0] = qc[u][1]; qc[u][1] = qc[u][2]; qc[u][2] = qc[u][3]; qc[u][3] = cv;
             qx[u][0] = qx[u][1]; qx[u][1] = qx[u][2]; qx[u][2] = qx[u][3];
             int gk = (D == 3) ? P : 0, gjj = (D == 3) ? pos.gj[u] : P;
-            MSten<D, L0, CT> t;
-            if constexpr (L0) t = mst<D, L0, RX, RY, CT>(cv, cv, S, s, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
-            else { t.d = cv.d; }
-            CT xo = (((pos.gi[u] + gjj + gk) & 1) == 1 && t.d >= CT(1e-30)) ? mdiv<CT>(bv + CT(0), t.d) : CT(0);
-            qx[u][3] = xo;
-            S.x[s][pos.e[u]] = xo;
+            if constexpr (!XD) {
+                MSten<D, L0, CT> t;
+                if constexpr (L0) t = mst<D, L0, RX, RY, CT>(cv, cv, S, s, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                else { t.d = cv.d; }
+                CT xo = (((pos.gi[u] + gjj + gk) & 1) == 1 && t.d >= CT(1e-30)) ? mdiv<CT>(bv + CT(0), t.d) : CT(0);
+                qx[u][3] = xo;
+                S.x[s][pos.e[u]] = xo;
+            }
             if constexpr (!L0) { S.ca[s][pos.e[u]] = cv.cx; if (D == 3) S.cb[s][pos.e[u]] = cv.cy; }
         }
         issue(pfb, pfc, P + 2);
         __syncthreads();
+        if constexpr (XD) {   // odd cells of plane P: b/d with the formed diagonal
+#pragma unroll
+            for (int u = 0; u < PPT; u++) {
+                int gk = (D == 3) ? P : 0, gjj = (D == 3) ? pos.gj[u] : P;
+                // the outer ring has no -x/-y neighbour in the region: stored diagonal
+                const CT d = pos.r1[u] ? mst<D, L0, RX, RY, CT, XD>(qc[u][3], qc[u][2], S, s, pos.e[u], A.c, g,
+                                                                    pos.gi[u], gjj, gk).d
+                                       : qc[u][3].d;
+                CT xo = (((pos.gi[u] + gjj + gk) & 1) == 1 && d >= CT(1e-30)) ? mdiv<CT>(qb[u][2] + CT(0), d) : CT(0);
+                qx[u][3] = xo;
+                S.x[s][pos.e[u]] = xo;
+            }
+        }
         // S2: forward sweep, even colour, plane Q = P-1 on the 1-ring region
         const int Q = P - 1;
         if (Q >= m0 - 1) {
@@ -224,7 +250,7 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
                 if (!pos.r1[u]) continue;
                 int gk = (D == 3) ? Q : 0, gjj = (D == 3) ? pos.gj[u] : Q;
                 if (((pos.gi[u] + gjj + gk) & 1) != 0) continue;
-                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT>(qc[u][2], qc[u][1], S, sq, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT, XD>(qc[u][2], qc[u][1], S, sq, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
                 if (!(t.d >= CT(1e-30))) continue;
                 const int e = pos.e[u];
                 CT nb = msum<D>(t, S.x[sq][e + 1], S.x[sq][e - 1], D == 3 ? S.x[sq][e + RX] : CT(0),
@@ -244,7 +270,7 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
             for (int u = 0; u < PPT; u++) {
                 if (!pos.r0[u]) continue;
                 int gk = (D == 3) ? R : 0, gjj = (D == 3) ? pos.gj[u] : R;
-                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT>(qc[u][1], qc[u][0], S, sr, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT, XD>(qc[u][1], qc[u][0], S, sr, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
                 const int e = pos.e[u];
                 RT res = RT(0);
                 if (t.fluid) {
@@ -301,6 +327,9 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
                                               const int* skip, double* partials, int bx, int by, int bz) {
     using CT = typename CTOf<VB>::type;
     using Cell = MCell<L0, CT>;
+    // BF16 records: as in d_down_m, the entering plane's odd b/d and the
+    // prolongation move behind the plane's barrier
+    constexpr bool XD = !L0 && std::is_same<C4, BF16x4>::value;
     constexpr int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1;
     static_assert(RX * RY / PPT == 256, "256 threads");
     if (skip && *skip) return;
@@ -345,7 +374,12 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
     for (int u = 0; u < PPT; u++) {
         for (int q = 0; q < 3; q++) qb[u][q] = CT(0);
         for (int q = 0; q < 4; q++) { qx[u][q] = CT(0); qc[u][q] = mload_cell<L0, CT, C4, D>(A.nbr, A.rec, 0, false); }
+        if constexpr (XD) {   // the -z coupling of the first halo plane
+            const bool v3 = pos.in[u] && m0 - 3 >= 1 && m0 - 3 <= nm;
+            qc[u][3] = mload_cell<L0, CT, C4, D>(A.nbr, A.rec, pos.off[u] + (long)(m0 - 3) * ms, v3);
+        }
     }
+    CT xcs[PPT];   // coarse x of the entering plane (XD: used behind the barrier)
     double red[3] = {0.0, 0.0, 0.0};
 
     auto body = [&](int P, VB (&pfb)[PPT], Cell (&pfc)[PPT], CT (&pfx)[PPT]) {
@@ -360,17 +394,35 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
             qc[u][0] = qc[u][1]; qc[u][1] = qc[u][2]; qc[u][2] = qc[u][3]; qc[u][3] = cv;
             qx[u][0] = qx[u][1]; qx[u][1] = qx[u][2]; qx[u][2] = qx[u][3];
             int gk = (D == 3) ? P : 0, gjj = (D == 3) ? pos.gj[u] : P;
-            MSten<D, L0, CT> t;
-            if constexpr (L0) t = mst<D, L0, RX, RY, CT>(cv, cv, S, s, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
-            else { t.d = cv.d; t.fluid = cv.d >= CT(0); }
-            CT v = (((pos.gi[u] + gjj + gk) & 1) == 1 && t.d >= CT(1e-30)) ? mdiv<CT>(bv + CT(0), t.d) : CT(0);
-            if (t.d >= CT(0)) v = v + CT(2) * xcv;
-            qx[u][3] = v;
-            S.x[s][pos.e[u]] = v;
+            if constexpr (!XD) {
+                MSten<D, L0, CT> t;
+                if constexpr (L0) t = mst<D, L0, RX, RY, CT>(cv, cv, S, s, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                else { t.d = cv.d; t.fluid = cv.d >= CT(0); }
+                CT v = (((pos.gi[u] + gjj + gk) & 1) == 1 && t.d >= CT(1e-30)) ? mdiv<CT>(bv + CT(0), t.d) : CT(0);
+                if (t.d >= CT(0)) v = v + CT(2) * xcv;
+                qx[u][3] = v;
+                S.x[s][pos.e[u]] = v;
+            } else {
+                xcs[u] = xcv;
+            }
             if constexpr (!L0) { S.ca[s][pos.e[u]] = cv.cx; if (D == 3) S.cb[s][pos.e[u]] = cv.cy; }
         }
         issue(pfb, pfc, pfx, P + 2);
         __syncthreads();
+        if constexpr (XD) {   // plane P: odd b/d with the formed diagonal, + 2 x_coarse on fluid cells
+#pragma unroll
+            for (int u = 0; u < PPT; u++) {
+                int gk = (D == 3) ? P : 0, gjj = (D == 3) ? pos.gj[u] : P;
+                // the outer ring has no -x/-y neighbour in the region: stored diagonal
+                const CT d = pos.r1[u] ? mst<D, L0, RX, RY, CT, XD>(qc[u][3], qc[u][2], S, s, pos.e[u], A.c, g,
+                                                                    pos.gi[u], gjj, gk).d
+                                       : qc[u][3].d;
+                CT v = (((pos.gi[u] + gjj + gk) & 1) == 1 && d >= CT(1e-30)) ? mdiv<CT>(qb[u][2] + CT(0), d) : CT(0);
+                if (d >= CT(0)) v = v + CT(2) * xcs[u];
+                qx[u][3] = v;
+                S.x[s][pos.e[u]] = v;
+            }
+        }
         // S2: reverse sweep, even colour, plane Q = P-1 on the 1-ring region
         const int Q = P - 1;
         if (Q >= m0 - 1) {
@@ -380,7 +432,7 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
                 if (!pos.r1[u]) continue;
                 int gk = (D == 3) ? Q : 0, gjj = (D == 3) ? pos.gj[u] : Q;
                 if (((pos.gi[u] + gjj + gk) & 1) != 0) continue;
-                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT>(qc[u][2], qc[u][1], S, sq, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT, XD>(qc[u][2], qc[u][1], S, sq, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
                 if (!(t.d >= CT(1e-30))) continue;
                 const int e = pos.e[u];
                 CT nb = msum<D>(t, S.x[sq][e + 1], S.x[sq][e - 1], D == 3 ? S.x[sq][e + RX] : CT(0),
@@ -399,7 +451,7 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
             for (int u = 0; u < PPT; u++) {
                 if (!pos.r0[u] || !pos.in[u]) continue;
                 int gk = (D == 3) ? R : 0, gjj = (D == 3) ? pos.gj[u] : R;
-                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT>(qc[u][1], qc[u][0], S, sr, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
+                MSten<D, L0, CT> t = mst<D, L0, RX, RY, CT, XD>(qc[u][1], qc[u][0], S, sr, pos.e[u], A.c, g, pos.gi[u], gjj, gk);
                 const int e = pos.e[u];
                 CT v = qx[u][1];
                 if (((pos.gi[u] + gjj + gk) & 1) == 1 && t.d >= CT(1e-30)) {
