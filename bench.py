@@ -375,6 +375,9 @@ def bench_serve(args):
         r = r.clone()
         if keep is not None:
             keep.append(r)
+        # the rhs is produced on torch's stream and read on the server's:
+        # it must be complete before the server can admit the request
+        torch.cuda.current_stream().synchronize()
         req = srv.submit(k, r.data_ptr(), p[k].data_ptr(), 1 if frames[k] else 0, 500, tol)
         frames[k] += 1
         return req, r
@@ -428,6 +431,7 @@ def bench_serve(args):
             iters.append(st.iters)
             if d in sample:
                 sample[d] = sample[d] + (p[k].clone(),)
+                torch.cuda.current_stream().synchronize()   # copied before the next frame of k can write p[k]
         if not live and nxt < nreq:
             time.sleep(max(0.0, min(0.002, arr[nxt] - (time.perf_counter() - t_start))))
     wall = time.perf_counter() - t_start

@@ -216,7 +216,11 @@ void uaamg_server_destroy(uaamg_server* s);
 const char* uaamg_server_last_error(const uaamg_server* s);
 /* Register simulation `sim` with its grid (host mask, copied). */
 int uaamg_server_register(uaamg_server* s, int sim, const uaamg_grid* g);
-/* Queue one solve of `sim` (device arrays); *req = request id. */
+/* Queue one solve of `sim` (device arrays); *req = request id.  The server
+ * reads rhs_dev and p_dev on its own stream from the step() that admits the
+ * request: work that writes them on another stream must be complete before
+ * submit (synchronize that stream).  p_dev holds the result once step()
+ * reports the request done. */
 int uaamg_server_submit(uaamg_server* s, int sim, const double* rhs_dev, double* p_dev, int warm_start,
                         int max_iter, double tol, int* req);
 /* Admit, run one chunk, complete: done[0..*n_done) = finished request ids. */
