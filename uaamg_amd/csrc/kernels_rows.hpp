@@ -650,10 +650,14 @@ template <typename C4> struct RecE;
 template <> struct RecE<float4> { using E = float; };
 template <> struct RecE<BF16x4> { using E = __bf16; };
 
+// BF16 records: the diagonal is formed from the six couplings (snap_diag)
+template <typename C4> constexpr bool rec_xd() { return std::is_same<C4, BF16x4>::value; }
+
 template <int CPL, int RW, typename C4>
 struct RecPf {            // one plane's loads
     float b[RW][CPL];
     C4 rec[RW][CPL];
+    C4 up[rec_xd<C4>() ? CPL : 1];   // BF16: the row above the first row (its cy is the -y coupling)
 };
 
 // LDS of the record kernels: the row exchange, and the cy of each wave's last
@@ -684,7 +688,14 @@ __device__ __forceinline__ void rec_issue(RecPf<CPL, RW, C4>& f, const SV* __res
 #pragma unroll
         for (int q = 0; q < CPL; q++) f.rec[t][q] = rec[th.roff[t] + po + q];
     }
+    if constexpr (rec_xd<C4>()) {
+        const int ju = min(max(th.jrow[0] - 1, 0), g.ny + 1);
+        const int ru = min(th.i0, g.nx + 1) + ju * (int)g.sx;
+#pragma unroll
+        for (int q = 0; q < CPL; q++) f.up[q] = rec[ru + po + q];
+    }
 }
+
 
 // per-plane register state of the record kernels
 template <int CPL, int RW>
@@ -693,11 +704,32 @@ struct RecQ {
     float Cx[4][RW][CPL], Cy[4][RW][CPL], Cz[4][RW][CPL];
 };
 
+// BF16 records: the -z coupling of the first halo plane (plane m0 - 3's cz)
+// into the slot before it (slot 2: plane m0 - 2 has slot 3)
+template <int CPL, int NW, int RW, typename C4>
+__device__ __forceinline__ void rec_prev_cz(RecQ<CPL, RW>& Q, const C4* __restrict__ rec,
+                                            const RecThread<CPL, NW, RW>& th, const Geo& g) {
+    if constexpr (rec_xd<C4>()) {
+        const int P = th.m0 - 3;
+        const int po = th.plane(g, P);
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const bool ok = P >= 1 && th.rowok[t] && th.okx[q];
+                Q.Cz[2][t][q] = ok ? (float)rec[th.roff[t] + po + q].z : 0.0f;
+            }
+    } else {
+        (void)Q; (void)rec; (void)th; (void)g;
+    }
+}
+
 // plane P (slot S) enters: unpack the prefetched loads; the last row's cy
 // goes to the LDS slot S for the wave below
 template <int S, int CPL, int NW, int RW, typename C4>
 __device__ __forceinline__ void rec_enter(RecQ<CPL, RW>& Q, const RecPf<CPL, RW, C4>& f, const RecThread<CPL, NW, RW>& th,
                                           bool planeok, RecLDS<CPL, NW, RW>& L) {
+    float dd[RW][CPL];
 #pragma unroll
     for (int t = 0; t < RW; t++)
 #pragma unroll
@@ -705,12 +737,33 @@ __device__ __forceinline__ void rec_enter(RecQ<CPL, RW>& Q, const RecPf<CPL, RW,
             const bool ok = planeok && th.rowok[t] && th.okx[q];
             const C4 r = f.rec[t][q];
             const float d = ok ? (float)r.w : -1.0f;
+            dd[t][q] = d;
             Q.Cx[S][t][q] = ok ? (float)r.x : 0.0f;
             Q.Cy[S][t][q] = ok ? (float)r.y : 0.0f;
             Q.Cz[S][t][q] = ok ? (float)r.z : 0.0f;
-            Q.Iq[S][t][q] = d >= 1e-30f ? __builtin_amdgcn_rcpf(d) : 0.0f;
             Q.Rq[S][t][q] = d >= 0.0f ? f.b[t][q] : 0.0f;
         }
+    if constexpr (rec_xd<C4>()) {
+        // the diagonal from the six couplings: -x from the previous cell (the
+        // previous lane for cell 0), -y from the row above (loaded for the
+        // first row), -z from the previous plane's slot
+        constexpr int SP = (S + 3) & 3;
+#pragma unroll
+        for (int t = 0; t < RW; t++) {
+            const float cl0 = lane_from_left(Q.Cx[S][t][CPL - 1]);
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                const float cxm = (q > 0) ? Q.Cx[S][t][q > 0 ? q - 1 : 0] : cl0;
+                const float cym = (t > 0) ? Q.Cy[S][t > 0 ? t - 1 : 0][q] : (float)f.up[q].y;
+                const float sum = Q.Cx[S][t][q] + cxm + Q.Cy[S][t][q] + cym + Q.Cz[S][t][q] + Q.Cz[SP][t][q];
+                if (dd[t][q] >= 0.0f) dd[t][q] = snap_diag<C4, float>(dd[t][q], sum);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RW; t++)
+#pragma unroll
+        for (int q = 0; q < CPL; q++) Q.Iq[S][t][q] = dd[t][q] >= 1e-30f ? __builtin_amdgcn_rcpf(dd[t][q]) : 0.0f;
 #pragma unroll
     for (int q = 0; q < CPL; q++) L.cy[S][th.w][q][th.lane] = Q.Cy[S][RW - 1][q];
 }
@@ -794,6 +847,7 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
     for (int a = 0; a < RW / 2; a++)
 #pragma unroll
         for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
+    rec_prev_cz<CPL, NW, RW, C4>(Q, rec, th, g);
     rec_issue<CPL, NW, RW, SV, C4>(pf[1], b, rec, th, g, th.m0 - 2);
     rec_issue<CPL, NW, RW, SV, C4>(pf[0], b, rec, th, g, th.m0 - 1);
 
@@ -931,6 +985,7 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
 #pragma unroll
             for (int c = 0; c < CPL / 2; c++) pfx[slot][a][c] = (float)xc[coff[a][c] + cpo];
     };
+    rec_prev_cz<CPL, NW, RW, C4>(Q, rec, th, g);
     issue(1, th.m0 - 2);
     issue(0, th.m0 - 1);
 
