@@ -483,7 +483,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid side (the metric is quoted at 256)")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "bf16r", "fp16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4, help="CG iterations queued between host convergence checks")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
@@ -521,7 +521,7 @@ def main():
     import uaamg_amd as U
     from uaamg_amd import workloads as W
 
-    prec = {"fp32": U.FP32, "fp64": U.FP64, "bf16": U.MIXED_BF16, "fp16": U.MIXED_FP16}[args.precision]
+    prec = {"fp32": U.FP32, "fp64": U.FP64, "bf16": U.MIXED_BF16, "bf16r": U.MIXED_BF16R, "fp16": U.MIXED_FP16}[args.precision]
     t0 = time.perf_counter()
     pb = W.plume3d(seed=0, n=args.n)
     log(f"[bench] generated {pb.name} in {time.perf_counter() - t0:.1f}s, solid fraction {pb.solid.mean():.3f}")
@@ -588,7 +588,7 @@ def main():
         if n_k[c]:
             log(f"[bench] {KNAMES[c]:>10s}: {ms_k[c] / n_k[c] * 1e3:8.1f} us/launch x {int(n_k[c] / args.steps)} per solve")
     sv = 8 if prec == U.FP64 else 4
-    sl = {U.FP64: 8, U.FP32: 4}.get(prec, 2)
+    sl = {U.FP64: 8, U.FP32: 4, U.MIXED_BF16R: 4}.get(prec, 2)   # coarse-level vector element size
     mode = {0: "plain", 1: "pmv", 2: "split"}[st.kernel_set]   # the level-0 kernel set the solves ran
     # the dominant kernel on the solve's critical path: in the split form the
     # x update runs on a second stream beside the coarse levels, so it is
@@ -613,7 +613,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "fp32", "fp64": "fp64", "bf16": "fp32+bf16", "fp16": "fp32+fp16"}[args.precision],
+        "dtype": {"fp32": "fp32", "fp64": "fp64", "bf16": "fp32+bf16", "bf16r": "fp32+bf16", "fp16": "fp32+fp16"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{pb.name}: 256^3 MAC pressure solve, dx=1/256, Neumann walls, floor slab y<0.1 + 8 seeded spheres, "
@@ -621,7 +621,8 @@ def main():
             "grid": list(pb.n),
             "relres": 1e-6,
             "precision": {"fp32": "FP32 vectors+levels, FP64 reductions", "fp64": "FP64",
-                          "bf16": "FP32 CG, BF16 levels", "fp16": "FP32 CG, FP16 levels"}[args.precision],
+                          "bf16": "FP32 CG, BF16 levels", "bf16r": "FP32 CG and level vectors, BF16 level records",
+                          "fp16": "FP32 CG, FP16 levels"}[args.precision],
             "iters": k_med,
             "hbm_gbs_solve": round(hbm_gbs, 1),
             "alg_bytes_per_solve": alg / args.steps,

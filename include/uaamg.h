@@ -37,7 +37,8 @@ enum {
     UAAMG_FP64 = 0,            /* everything FP64: parity mode */
     UAAMG_FP32 = 1,            /* plain form: FP32 vectors and levels, FP64 reductions/scalars */
     UAAMG_MIXED_BF16 = 2,      /* FP32 CG; 3D coarse levels stored BF16 (FP32 arithmetic); 2D levels stay FP32 */
-    UAAMG_MIXED_FP16 = 3       /* FP32 CG; coarse levels stored FP16 (h^2-normalised), FP32 arithmetic */
+    UAAMG_MIXED_FP16 = 3,      /* FP32 CG; coarse levels stored FP16 (h^2-normalised), FP32 arithmetic */
+    UAAMG_MIXED_BF16R = 4      /* FP32 CG and level vectors; 3D coarse-level stencil records stored BF16 */
 };
 
 enum { UAAMG_WALL_NEUMANN = 0, UAAMG_WALL_DIRICHLET = 1 };

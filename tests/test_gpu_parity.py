@@ -141,13 +141,15 @@ def test_amgpcg(gpu, N, prec):
 
 # ---- mixed form: BF16 level storage, FP32 CG, FP64 reductions ----
 
+@pytest.mark.parametrize("prec", [U.MIXED_BF16, U.MIXED_BF16R], ids=["bf16", "bf16r"])
 @pytest.mark.parametrize("name,pb", CASES, ids=[c[0] for c in CASES])
-def test_vcycle_bf16_levels(gpu, name, pb):
-    """BF16 coarse levels (8-bit mantissa couplings, b, x) perturb the
-    preconditioner at the 1e-2 level: it stays close to the FP64 V-cycle."""
+def test_vcycle_bf16_levels(gpu, name, pb, prec):
+    """BF16 coarse levels (8-bit mantissa couplings, b, x; or the couplings
+    only, with FP32 b and x) perturb the preconditioner at the 1e-2 level: it
+    stays close to the FP64 V-cycle."""
     rng = np.random.default_rng(11)
     r = cases.random_rhs(pb, rng)
-    c = ctx_for(pb, precision=U.MIXED_BF16)
+    c = ctx_for(pb, precision=prec)
     z = c.apply(r)
     zo = ob.uaamg_apply(pb, r)
     m = np.zeros(pb.N, bool); pb.interior(m)[...] = True
@@ -156,9 +158,10 @@ def test_vcycle_bf16_levels(gpu, name, pb):
     c.close()
 
 
-@pytest.mark.parametrize("flexible", [0, 1])
+@pytest.mark.parametrize("prec,flexible", [(U.MIXED_BF16, 0), (U.MIXED_BF16, 1), (U.MIXED_BF16R, 0)],
+                         ids=["bf16", "bf16_flexible", "bf16r"])
 @pytest.mark.parametrize("name,pb", PCG_CASES, ids=[c[0] for c in PCG_CASES])
-def test_pcg_bf16_levels(gpu, name, pb, flexible):
+def test_pcg_bf16_levels(gpu, name, pb, prec, flexible):
     """The CG residual, not the preconditioner, sets the accuracy: at relres
     1e-6 the pressure is within 2e-5 of the FP64 oracle (mean removed), the
     bar of the plain FP32 form; the iteration count may grow by the
@@ -166,7 +169,7 @@ def test_pcg_bf16_levels(gpu, name, pb, flexible):
     rng = np.random.default_rng(3)
     rhs = 1e3 * cases.random_rhs(pb, rng)
     po, res = ob.pcg(pb, rhs, 500, 1e-10, tol_mode=1)
-    c = ctx_for(pb, precision=U.MIXED_BF16, tol_mode=U.TOL_REL, flexible=flexible)
+    c = ctx_for(pb, precision=prec, tol_mode=U.TOL_REL, flexible=flexible)
     p, st = c.solve(rhs, 500, 1e-6)
     assert st.status == 0, U.status_name(st.status)
     assert st.iters <= 1.5 * res.iters + 5, (st.iters, res.iters)
@@ -178,7 +181,7 @@ def test_pcg_bf16_levels(gpu, name, pb, flexible):
 
 # ---- hipGraph replay of the CG iterations ----
 
-@pytest.mark.parametrize("prec", [U.FP64, U.FP32, U.MIXED_BF16])
+@pytest.mark.parametrize("prec", [U.FP64, U.FP32, U.MIXED_BF16, U.MIXED_BF16R])
 @pytest.mark.parametrize("name,pb", [c for c in PCG_CASES if c[0] in ("3d_24x20x18_aniso_solids", "2d_48x40_aniso_solids", "3d_40x36x34_walls_solids")],
                          ids=lambda v: v if isinstance(v, str) else "")
 def test_graph_replay_matches_eager(gpu, name, pb, prec):

@@ -115,3 +115,24 @@ def test_plume3d_host_check_schedules_agree(gpu):
     for every, graph, it, p in res[1:]:
         assert it == res[0][2], (every, graph, it, res[0][2])
         np.testing.assert_array_equal(p, res[0][3])
+
+
+def test_plume3d_bf16_records_form(gpu):
+    """BF16 stencil records with FP32 level vectors (MIXED_BF16R) at 128^3:
+    the FP32 form's iteration count (within 2) and its agreement with the
+    FP64 pressure; the BF16 b and x of MIXED_BF16 are what cost it
+    iterations (136 against 120 here)."""
+    pb = W.plume3d(seed=0, n=128)
+    out = {}
+    for prec in (U.FP64, U.FP32, U.MIXED_BF16R):
+        c = U.Context(precision=prec, tol_mode=U.TOL_REL, check_every=4)
+        c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+        p, st = c.solve(pb.rhs, 400, 1e-6 if prec != U.FP64 else 1e-9)
+        c.close()
+        assert st.status == 0, (prec, U.status_name(st.status))
+        out[prec] = (p, st.iters)
+    assert out[U.MIXED_BF16R][1] <= out[U.FP32][1] + 2, (out[U.MIXED_BF16R][1], out[U.FP32][1])
+    m = pb.fluid_mask()
+    a = out[U.MIXED_BF16R][0][m]; a = a - a.mean()
+    b = out[U.FP64][0][m]; b = b - b.mean()
+    assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max()

@@ -6,12 +6,12 @@ Python view used by tests and the benchmark.  There is no CPU fallback: if the
 library is missing or no gfx950 device is present, calls raise.
 """
 from .capi import (Lib, Context, Batch, Server, Stats, Opts, GridDesc, load, lib_path,
-                   FP64, FP32, MIXED_BF16, MIXED_FP16, TOL_ABS, TOL_REL,
+                   FP64, FP32, MIXED_BF16, MIXED_FP16, MIXED_BF16R, TOL_ABS, TOL_REL,
                    NEUMANN, DIRICHLET, status_name, OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NODEVICE,
                    ERR_ADMISSION, NOT_CONVERGED, BREAKDOWN, NONFINITE)
 from .build import build, build_host_tools
 
 __all__ = ["Lib", "Context", "Batch", "Server", "Stats", "Opts", "GridDesc", "load", "lib_path", "build", "build_host_tools",
-           "FP64", "FP32", "MIXED_BF16", "MIXED_FP16", "TOL_ABS", "TOL_REL",
+           "FP64", "FP32", "MIXED_BF16", "MIXED_FP16", "MIXED_BF16R", "TOL_ABS", "TOL_REL",
            "NEUMANN", "DIRICHLET", "status_name", "OK", "ERR_ARG", "ERR_HIP", "ERR_OOM", "ERR_NODEVICE",
            "ERR_ADMISSION", "NOT_CONVERGED", "BREAKDOWN", "NONFINITE"]

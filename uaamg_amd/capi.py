@@ -5,7 +5,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-FP64, FP32, MIXED_BF16, MIXED_FP16 = 0, 1, 2, 3
+FP64, FP32, MIXED_BF16, MIXED_FP16, MIXED_BF16R = 0, 1, 2, 3, 4
 TOL_ABS, TOL_REL = 0, 1
 NEUMANN, DIRICHLET = 0, 1
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NODEVICE, ERR_ADMISSION = 0, 1, 2, 3, 4, 5

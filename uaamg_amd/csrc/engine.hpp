@@ -150,11 +150,13 @@ Consts<CT> make_consts(int dim, const double h[3], const int wall[6]) {
     return c;
 }
 
-template <int D, typename T, typename S>
+// R: storage of the coarse levels' stencil records (default: S, the level
+// vectors' storage)
+template <int D, typename T, typename S, typename R = S>
 class Engine final : public EngineBase {
     using CT = typename CTOf<T>::type;
     using SC = typename CTOf<S>::type;
-    using C4 = typename Vec4<S>::type;
+    using C4 = typename Vec4<R>::type;
 
     struct Level {
         Geo g;
@@ -926,7 +928,7 @@ private:
     double alg_bytes(int iters) const {
         // canonical count of the reference algorithm's logical passes, each array read or written once per pass:
         // per iteration 13 s_v + 2 bytes (CG body) + level-0 V-cycle + coarse levels.
-        double sv = sizeof(T), sl = sizeof(S), sc = sizeof(S);
+        double sv = sizeof(T), sl = sizeof(S), sc = sizeof(R);
         double rho = (D == 3) ? 1.0 / 8.0 : 1.0 / 4.0, ch = (D == 3) ? 4 : 3;
         double N0 = (double)n_[0] * n_[1] * (D == 3 ? n_[2] : 1);
         double cg = 13 * sv + 2;

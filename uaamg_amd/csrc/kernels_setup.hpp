@@ -162,15 +162,16 @@ __global__ void k_level_diag(C4* __restrict__ coef, const S* __restrict__ cex, c
             double e = (double)cex[id];
             if (e != 0.0) d += e;
         }
-        if constexpr (sizeof(S) == 2) {
-            // 16-bit levels: round the diagonal up, so that it never drops below
+        using RE = typename std::remove_cv<typename std::remove_reference<decltype(coef[id].w)>::type>::type;
+        if constexpr (sizeof(RE) == 2) {
+            // 16-bit records: round the diagonal up, so that it never drops below
             // the sum of the (rounded) couplings -- the coarse operator stays
             // diagonally dominant, hence positive definite, and the V-cycle a
             // valid preconditioner
-            S dd = (S)d;
+            RE dd = (RE)d;
             if (d > 0.0 && (double)dd < d) {
                 uint16_t u = __builtin_bit_cast(uint16_t, dd);
-                dd = __builtin_bit_cast(S, (uint16_t)(u + 1));
+                dd = __builtin_bit_cast(RE, (uint16_t)(u + 1));
             }
             coef[id].w = dd;
         } else {

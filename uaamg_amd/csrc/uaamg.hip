@@ -25,21 +25,21 @@ __global__ void k_cast_all(const A* __restrict__ src, B* __restrict__ dst, long 
     for (; t < n; t += s) dst[t] = (B)src[t];
 }
 
-template <int D, typename T, typename S>
-void Engine<D, T, S>::cast_in_all(const double* src, T* dst) {
+template <int D, typename T, typename S, typename R>
+void Engine<D, T, S, R>::cast_in_all(const double* src, T* dst) {
     long n = g0_.N;
     k_cast_all<double, T><<<(unsigned)std::min<long>((n + 255) / 256, 4096), 256, 0, stream_>>>(src, dst, n);
     HIPCHK(hipGetLastError());
 }
-template <int D, typename T, typename S>
-void Engine<D, T, S>::cast_out_all(const T* src, double* dst) {
+template <int D, typename T, typename S, typename R>
+void Engine<D, T, S, R>::cast_out_all(const T* src, double* dst) {
     long n = g0_.N;
     k_cast_all<T, double><<<(unsigned)std::min<long>((n + 255) / 256, 4096), 256, 0, stream_>>>(src, dst, n);
     HIPCHK(hipGetLastError());
 }
 
-template <int D, typename T, typename S>
-void Engine<D, T, S>::allocate() {
+template <int D, typename T, typename S, typename R>
+void Engine<D, T, S, R>::allocate() {
     const int nl = nlevels();
     lv_.resize(nl);
     lv_[0].g = g0_;
@@ -72,8 +72,8 @@ void Engine<D, T, S>::allocate() {
     HIPCHK(hipStreamSynchronize(stream_));
 }
 
-template <int D, typename T, typename S>
-void Engine<D, T, S>::release() {
+template <int D, typename T, typename S, typename R>
+void Engine<D, T, S, R>::release() {
     (void)hipStreamSynchronize(stream_);
     for (void* p : allocs_) (void)hipFree(p);
     allocs_.clear();
@@ -83,8 +83,8 @@ void Engine<D, T, S>::release() {
     hist_ = nullptr;
 }
 
-template <int D, typename T, typename S>
-void Engine<D, T, S>::level_data(int level, uint8_t* solid, double* diag, double* cx, double* cy, double* cz, double* ex) {
+template <int D, typename T, typename S, typename R>
+void Engine<D, T, S, R>::level_data(int level, uint8_t* solid, double* diag, double* cx, double* cy, double* cz, double* ex) {
     const Geo g = shape_[level];
     const long N = g.N;
     std::vector<uint8_t> m(N);
@@ -204,7 +204,7 @@ extern "C" int uaamg_default_opts(uaamg_opts* o) {
 }
 
 static bool opts_valid(const uaamg_opts* o, std::string& why) {
-    if (o->precision < UAAMG_FP64 || o->precision > UAAMG_MIXED_FP16) { why = "precision"; return false; }
+    if (o->precision < UAAMG_FP64 || o->precision > UAAMG_MIXED_BF16R) { why = "precision"; return false; }
     if (o->nu_pre < 0 || o->nu_post < 0 || o->nu_pre > 16 || o->nu_post > 16) { why = "nu_pre/nu_post"; return false; }
     if (o->coarsest_side < 1) { why = "coarsest_side"; return false; }
     if (o->coarsest_sweeps < 0) { why = "coarsest_sweeps"; return false; }
@@ -297,6 +297,7 @@ static std::unique_ptr<EngineBase> make_engine(const uaamg_opts& o, const uaamg_
             // the level state, 2D solves with solids take 2-3x the iterations or
             // stagnate (measured on the batched mixed-size stream); 3D loses ~10 %
             case UAAMG_MIXED_BF16: return std::make_unique<Engine<2, float, float>>(o, n, h, g.wall, s);
+            case UAAMG_MIXED_BF16R: return std::make_unique<Engine<2, float, float>>(o, n, h, g.wall, s);
             default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
         }
     }
@@ -304,6 +305,7 @@ static std::unique_ptr<EngineBase> make_engine(const uaamg_opts& o, const uaamg_
         case UAAMG_FP64: return std::make_unique<Engine<3, double, double>>(o, n, h, g.wall, s);
         case UAAMG_FP32: return std::make_unique<Engine<3, float, float>>(o, n, h, g.wall, s);
         case UAAMG_MIXED_BF16: return std::make_unique<Engine<3, float, __bf16>>(o, n, h, g.wall, s);
+        case UAAMG_MIXED_BF16R: return std::make_unique<Engine<3, float, float, __bf16>>(o, n, h, g.wall, s);
         default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
     }
 }
@@ -541,7 +543,12 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     if (shape.empty()) return 0;
     // mirrors Engine::allocate (device bytes requested by a context)
     const size_t sv = (o->precision == UAAMG_FP64) ? 8 : 4;
-    const size_t sl = (o->precision == UAAMG_FP64) ? 8 : (o->precision == UAAMG_FP32 ? 4 : 2);
+    // coarse-level vectors (b, x, excess) and stencil records: 16-bit only in
+    // 3D (2D hierarchies keep FP32 levels under the 16-bit forms)
+    const bool d3 = g->dim == 3;
+    const size_t sl = (o->precision == UAAMG_FP64) ? 8 : ((o->precision == UAAMG_MIXED_BF16 && d3) ? 2 : 4);
+    const size_t sr = (o->precision == UAAMG_FP64) ? 8
+                    : (((o->precision == UAAMG_MIXED_BF16 || o->precision == UAAMG_MIXED_BF16R) && d3) ? 2 : 4);
     const Geo& g0 = shape[0];
     const size_t N0 = (size_t)g0.N, N4 = (N0 + 3) / 4 * 4;
     const size_t margin = (size_t)((g0.sx + 8 + 63) / 64) * 64;
@@ -552,7 +559,7 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     if (pmv) bytes += (size_t)(8 * ((((g0.ny + 15) / 16) * ((g0.nz + 3) / 4) + 7) / 8) + 64) * 8;   // p.Ap partials
     long maxb = std::max<long>(cells_launch(g0, g->dim).nblocks(), 2048);
     for (size_t l = 1; l < shape.size(); l++) {
-        bytes += (size_t)shape[l].N * (7 * sl + 1) + (size_t)kRowSlack * 6 * sl;
+        bytes += (size_t)shape[l].N * (3 * sl + 4 * sr + 1) + (size_t)kRowSlack * 6 * std::max(sl, sr);
         maxb = std::max(maxb, cells_launch(shape[l], g->dim).nblocks());
     }
     // fused level-0 launch (at most) and the row kernels' workgroups
