@@ -446,7 +446,7 @@ __global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
 // set while it runs).  With NB = false the mask is not read: for a p that is
 // exactly 0 on every non-fluid cell and ghost (the operator pass's p), the
 // update leaves those cells of x unchanged by itself.
-template <typename T, bool NB = true>
+template <typename T, bool NB = true, bool NT = false>
 __global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint8_t* __restrict__ nbr, long N4,
                           const Ctrl* ctrl) {
     const double alpha = ctrl->xalpha;
@@ -459,11 +459,18 @@ __global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint
             w = *(const uint32_t*)(nbr + c);
             if (!(w & 0x01010101u)) continue;
         }
-        vec4<T> xv = *(const vec4<T>*)(x + c), pv = *(const vec4<T>*)(p + c);
+        vec4<T> xv, pv;
+        if constexpr (NT) {   // streaming: no reuse, keep the caches for the kernels beside it
+            xv = __builtin_nontemporal_load((const vec4<T>*)(x + c));
+            pv = __builtin_nontemporal_load((const vec4<T>*)(p + c));
+        } else {
+            xv = *(const vec4<T>*)(x + c); pv = *(const vec4<T>*)(p + c);
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++)
             if (!NB || (nbyte(w, q) & NB_FLUID)) xv[q] = (T)((double)xv[q] + alpha * (double)pv[q]);
-        *(vec4<T>*)(x + c) = xv;
+        if constexpr (NT) __builtin_nontemporal_store(xv, (vec4<T>*)(x + c));
+        else *(vec4<T>*)(x + c) = xv;
     }
 }
 
