@@ -121,12 +121,14 @@ def test_fused_warm_start_and_repeat(gpu, case):
 
 
 @pytest.mark.parametrize("env", [{"UAAMG_XNB": "1"}, {"UAAMG_XFORK": "0", "UAAMG_XBLOCKS": "2048"},
-                                 {"UAAMG_XFORK": "99", "UAAMG_XTHREADS": "512"}, {"UAAMG_XNT": "0"}],
+                                 {"UAAMG_XFORK": "99", "UAAMG_XTHREADS": "512"},
+                                 {"UAAMG_XNT": "0", "UAAMG_PNT": "0", "UAAMG_DNT": "0", "UAAMG_UNT": "0"}],
                          ids=["mask", "fork0_2048", "fork_tail_512", "temporal"])
 def test_fused_x_update_forms(gpu, env):
     """The side-stream x update (x += alpha p, elementwise) in its other forms
-    (mask read, other fork points and launch shapes, plain instead of
-    nontemporal memory operations) gives bit-identical
+    (mask read, other fork points and launch shapes), and plain instead of
+    nontemporal loads and stores in it and the level-0 kernels, give
+    bit-identical
     pressure to the default form: only where and how the same FP64 updates
     run changes."""
     name, pb = FUSED[L1REC]

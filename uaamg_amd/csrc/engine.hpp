@@ -505,7 +505,11 @@ public:
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
             if (fr_old_) {   // alpha from the operator's partials; r = r_old - alpha Ap written to r; r.r -> convergence
-                tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S> : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
+                // nontemporal loads of r_old and Ap (last use of both; UAAMG_DNT=0:
+                // plain): 34.5 vs 33.5 solves/s at 256^3 in three paired runs
+                static const bool dnt = [] { const char* e = std::getenv("UAAMG_DNT"); return !(e && e[0] == '0'); }();
+                tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? (dnt ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S, true> : k_rows_down0r<4, ROWS_NW, ROWS_RW, S>)
+                                                              : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
                         rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const double*)partials_m_,
                         (int)pmv_blocks_, ctrl_);
@@ -518,7 +522,11 @@ public:
     }
     void rows_up0(const T* r, Level& C, T* z, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? k_rows_up0<4, ROWS_NW, ROWS_RW, S> : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
+            // nontemporal loads of r (UAAMG_UNT=0: plain): 34.6 vs 33.9 solves/s
+            // at 256^3 in three paired runs
+            static const bool unt = [] { const char* e = std::getenv("UAAMG_UNT"); return !(e && e[0] == '0'); }();
+            tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? (unt ? k_rows_up0<4, ROWS_NW, ROWS_RW, S, true> : k_rows_up0<4, ROWS_NW, ROWS_RW, S>)
+                                                        : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
             // with the split update the next operator pass does this finish
             if (!fr_) k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
@@ -796,7 +804,11 @@ public:
             // them writes its r.r partials while other workgroups still read
             // (with the split update it also finishes the level-0 up kernel's z sums)
             if constexpr (D == 3 && std::is_same<T, float>::value) {
-                auto kern = fr_ ? (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, true> : k_rows_pmv<2, PMV_NW, PMV_RW, true>)
+                // nontemporal loads of z and p_old (last use of both; UAAMG_PNT=0:
+                // plain): 33.6 vs 33.0 solves/s at 256^3 in three paired runs
+                static const bool pnt = [] { const char* e = std::getenv("UAAMG_PNT"); return !(e && e[0] == '0'); }();
+                auto kern = fr_ ? (rows_cpl_ == 4 ? (pnt ? k_rows_pmv<4, PMV_NW, PMV_RW, true, true> : k_rows_pmv<4, PMV_NW, PMV_RW, true>)
+                                                  : k_rows_pmv<2, PMV_NW, PMV_RW, true>)
                                 : (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, false> : k_rows_pmv<2, PMV_NW, PMV_RW, false>);
                 tlaunch(timer_, UAAMG_K_MATVEC, kern, dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_,
                         (const float*)pold, (float*)pnew, (float*)Ap_, (const uint8_t*)nbr0_, g0_, ptab_, pgeo_,

@@ -51,7 +51,7 @@ struct PmvLDS {
 // and beta; workgroup 0 stores them.  rsold and the first-iteration flag are
 // read here and only written by the next kernel (the level-0 down kernel),
 // so no workgroup can see a value of this launch.
-template <int CPL, int NW, int RW, bool FZ = false>
+template <int CPL, int NW, int RW, bool FZ = false, bool NT = false>
 __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                            float* __restrict__ pnew, float* __restrict__ Ap,
                                            const uint8_t* __restrict__ nbr, const Geo& g, const PmvTab& K,
@@ -76,8 +76,10 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
         const int po = th.plane(g, P);
 #pragma unroll
         for (int t = 0; t < RW; t++) {
-            row_load<CPL>(z, nbr, th.roff[t] + po, th.okx, pfz[slot][t], pfn[slot][t]);
-            const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
+            row_load<CPL, NT>(z, nbr, th.roff[t] + po, th.okx, pfz[slot][t], pfn[slot][t]);
+            typename RowVec<CPL>::V v;
+            if constexpr (NT) v = __builtin_nontemporal_load((const typename RowVec<CPL>::V*)(pold + th.roff[t] + po));
+            else v = *(const typename RowVec<CPL>::V*)(pold + th.roff[t] + po);
 #pragma unroll
             for (int q = 0; q < CPL; q++) pfp[slot][t][q] = v[q];
         }
@@ -216,13 +218,13 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     block_store<1>(acc, partials);
 }
 
-template <int CPL, int NW, int RW, bool FZ = false>
+template <int CPL, int NW, int RW, bool FZ = false, bool NT = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                                       float* __restrict__ pnew, float* __restrict__ Ap,
                                                       const uint8_t* __restrict__ nbr, Geo g, PmvTab K,
                                                       RowGeo rg, double* partials, Ctrl* ctrl,
                                                       const double* zp, int nzp) {
-    d_rows_pmv<CPL, NW, RW, FZ>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x),
+    d_rows_pmv<CPL, NW, RW, FZ, NT>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x),
                                 zp, nzp);
 }
 

@@ -102,10 +102,12 @@ template <> struct RowVec<2> {
     typedef float V __attribute__((ext_vector_type(2), aligned(4)));
     typedef uint16_t B __attribute__((aligned(1)));
 };
-template <int CPL>
+template <int CPL, bool NT = false>
 __device__ __forceinline__ void row_load(const float* __restrict__ r, const uint8_t* __restrict__ nbr, int id,
                                          const bool (&okx)[CPL], float (&vr)[CPL], uint8_t (&vn)[CPL]) {
-    const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(r + id);
+    typename RowVec<CPL>::V v;
+    if constexpr (NT) v = __builtin_nontemporal_load((const typename RowVec<CPL>::V*)(r + id));
+    else v = *(const typename RowVec<CPL>::V*)(r + id);
     const uint32_t b = *(const typename RowVec<CPL>::B*)(nbr + id);
 #pragma unroll
     for (int q = 0; q < CPL; q++) {
@@ -208,7 +210,7 @@ struct RUpd {
     Ctrl* ctrl = nullptr;
 };
 
-template <int CPL, int NW, int RW, typename VC, bool FR = false>
+template <int CPL, int NW, int RW, typename VC, bool FR = false, bool NT = false>
 __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
                                              const RowTab& K, const RowGeo& rg, VC* __restrict__ bc, const Geo& cg,
                                              const int* skip, int tile, const RUpd& u = RUpd{}) {
@@ -278,9 +280,11 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
         const int po = th.plane(g, P);
 #pragma unroll
         for (int t = 0; t < RW; t++) {
-            row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
+            row_load<CPL, NT>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
             if constexpr (FR) {
-                const typename RowVec<CPL>::V v = *(const typename RowVec<CPL>::V*)(u.Ap + th.roff[t] + po);
+                typename RowVec<CPL>::V v;
+                if constexpr (NT) v = __builtin_nontemporal_load((const typename RowVec<CPL>::V*)(u.Ap + th.roff[t] + po));
+                else v = *(const typename RowVec<CPL>::V*)(u.Ap + th.roff[t] + po);
 #pragma unroll
                 for (int q = 0; q < CPL; q++) pfa[slot][t][q] = v[q];
             }
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
 
 // down0 with the fused residual update (r_old -> r_new, r.r partials) and
 // the operator's finish (alpha from the p.Ap partials)
-template <int CPL, int NW, int RW, typename VC>
+template <int CPL, int NW, int RW, typename VC, bool NT = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict__ rold, const uint8_t* __restrict__ nbr,
                                                          Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
                                                          const int* skip, const float* __restrict__ Ap,
@@ -442,13 +446,13 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict
                                                          const double* pap, int npap, Ctrl* ctrl) {
     RUpd u;
     u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.pap = pap; u.npap = npap; u.ctrl = ctrl;
-    d_rows_down0<CPL, NW, RW, VC, true>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
+    d_rows_down0<CPL, NW, RW, VC, true, NT>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
 }
 
 // ------------------------------------------------------------------ up
 // r, coarse x -> z (prolongation + reverse sweep), with sum z, sum r z, sum r
 // over fluid cells reduced per workgroup into partials.
-template <int CPL, int NW, int RW, typename VCX>
+template <int CPL, int NW, int RW, typename VCX, bool NT = false>
 __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
                                            const RowTab& K, const RowGeo& rg, const VCX* __restrict__ xc, const Geo& cg,
                                            float* __restrict__ z, const int* skip, double* partials, int tile) {
@@ -491,7 +495,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
         auto issue = [&](int slot, int P) {
             const int po = th.plane(g, P);
 #pragma unroll
-            for (int t = 0; t < RW; t++) row_load<CPL>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
+            for (int t = 0; t < RW; t++) row_load<CPL, NT>(r, nbr, th.roff[t] + po, th.okx, pfr[slot][t], pfn[slot][t]);
             // coarse x of the parents; masked where it is used (S1), so the
             // loads stay in flight
             const int kcc = (P + 1) >> 1;
@@ -612,11 +616,11 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
     block_store<3>(red, partials);
 }
 
-template <int CPL, int NW, int RW, typename VCX>
+template <int CPL, int NW, int RW, typename VCX, bool NT = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
                                                       Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
                                                       float* __restrict__ z, const int* skip, double* partials) {
-    d_rows_up0<CPL, NW, RW, VCX>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
+    d_rows_up0<CPL, NW, RW, VCX, NT>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
 }
 
 // ================================================================== record levels
