@@ -283,68 +283,6 @@ def bench_project(args):
     return res
 
 
-def _serve_sims(nsim, seed, dev):
-    """Simulations of the served stream: half 2D (side log-uniform 64..2048),
-    half 3D (side log-uniform 32..256), all walls Neumann, up to 3 disjoint
-    solid balls away from the walls (the fluid stays connected), generated on
-    the device."""
-    import torch
-    rng = np.random.default_rng(seed)
-    sims = []
-    for k in range(nsim):
-        dim = 2 if k % 2 == 0 else 3
-        lo, hi = (64, 2048) if dim == 2 else (32, 256)
-        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
-        n = (side,) * dim
-        shape = tuple(v + 2 for v in n)[::-1]
-        ax = [((torch.arange(s, device=dev, dtype=torch.float32) - 0.5) / (s - 2)) for s in shape]
-        X = torch.meshgrid(*ax, indexing="ij")
-        solid = torch.zeros(shape, dtype=torch.bool, device=dev)
-        balls = []
-        for _ in range(int(rng.integers(0, 4))):
-            for _try in range(20):
-                r = rng.uniform(0.04, 0.1)
-                c = rng.uniform(r + 0.05, 1 - r - 0.05, dim)
-                if all(np.linalg.norm(c - c2) > r + r2 + 0.05 for c2, r2 in balls):
-                    balls.append((c, r))
-                    break
-        for c, r in balls:
-            d2 = sum((X[q] - float(c[dim - 1 - q])) ** 2 for q in range(dim))
-            solid |= d2 < r * r
-        inner = (slice(1, -1),) * dim
-        fluid = torch.zeros(shape, dtype=torch.bool, device=dev)
-        fluid[inner] = True
-        fluid &= ~solid
-        # smooth rhs: a few Gaussian sources and sinks, zero mean over the fluid
-        rhs = torch.zeros(shape, dtype=torch.float64, device=dev)
-        for _ in range(6):
-            c = rng.uniform(0.1, 0.9, dim)
-            s = rng.uniform(0.05, 0.2)
-            rhs += rng.uniform(-1e3, 1e3) * torch.exp(-sum((X[q].double() - float(c[dim - 1 - q])) ** 2 for q in range(dim)) / s ** 2)
-        rhs[~fluid] = 0
-        rhs[fluid] -= rhs[fluid].mean()
-        sims.append({"n": n, "h": tuple(1.0 / v for v in n), "solid": solid.reshape(-1).to(torch.uint8).cpu().numpy(),
-                     "fluid": fluid.reshape(-1), "rhs": rhs.reshape(-1), "shape": shape, "dim": dim})
-    return sims
-
-
-def _next_frame(sim, rng):
-    """Frame t+1: frame t's rhs advected by a smooth (one-cell) shift plus 5 % new noise."""
-    import torch
-    r = sim["rhs"].reshape(sim["shape"])
-    ax = int(rng.integers(0, sim["dim"]))
-    r = torch.roll(r, shifts=int(rng.choice([-1, 1])), dims=ax)
-    noise = torch.randn(r.shape, dtype=torch.float64, device=r.device)
-    noise = noise * (0.05 * float(r.abs().max()))
-    r = r + noise
-    r = r.reshape(-1)
-    f = sim["fluid"]
-    r[~f] = 0
-    r[f] -= r[f].mean()
-    sim["rhs"] = r
-    return r
-
-
 def bench_serve(args):
     """Served stream: 200 concurrent simulations submit one pressure solve per
     frame (relres 1e-5, warm start from their previous pressure).  Closed
@@ -354,10 +292,11 @@ def bench_serve(args):
     same rhs (the FP64 path matches the FP64 oracle to 1e-9)."""
     import torch
     import uaamg_amd as U
+    from uaamg_amd import workloads as W
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
     nsim = args.serve_sims
-    sims = _serve_sims(nsim, 7, dev)
+    sims = W.serve_sims(nsim, 7, dev)
     cells = sum(int(np.prod(s["n"])) for s in sims)
     log(f"[serve] generated {nsim} simulations ({cells / 1e6:.1f} M cells) in {time.perf_counter() - t0:.1f}s")
     budget = int(args.serve_budget_gb * (1 << 30))
@@ -371,7 +310,7 @@ def bench_serve(args):
     frames = [0] * nsim
 
     def submit(k, keep=None):
-        r = _next_frame(sims[k], rng) if frames[k] else sims[k]["rhs"]
+        r = W.next_frame(sims[k], rng) if frames[k] else sims[k]["rhs"]
         r = r.clone()
         if keep is not None:
             keep.append(r)
@@ -483,7 +422,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid side (the metric is quoted at 256)")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "bf16r", "fp16"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "bf16", "bf16r", "mixed64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=4, help="CG iterations queued between host convergence checks")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (stderr)")
@@ -521,7 +460,7 @@ def main():
     import uaamg_amd as U
     from uaamg_amd import workloads as W
 
-    prec = {"fp32": U.FP32, "fp64": U.FP64, "bf16": U.MIXED_BF16, "bf16r": U.MIXED_BF16R, "fp16": U.MIXED_FP16}[args.precision]
+    prec = {"fp32": U.FP32, "fp64": U.FP64, "bf16": U.MIXED_BF16, "bf16r": U.MIXED_BF16R, "mixed64": U.MIXED_F64}[args.precision]
     t0 = time.perf_counter()
     pb = W.plume3d(seed=0, n=args.n)
     log(f"[bench] generated {pb.name} in {time.perf_counter() - t0:.1f}s, solid fraction {pb.solid.mean():.3f}")
@@ -587,8 +526,8 @@ def main():
     for c in range(8):
         if n_k[c]:
             log(f"[bench] {KNAMES[c]:>10s}: {ms_k[c] / n_k[c] * 1e3:8.1f} us/launch x {int(n_k[c] / args.steps)} per solve")
-    sv = 8 if prec == U.FP64 else 4
-    sl = {U.FP64: 8, U.FP32: 4, U.MIXED_BF16R: 4}.get(prec, 2)   # coarse-level vector element size
+    sv = 8 if prec in (U.FP64, U.MIXED_F64) else 4
+    sl = {U.FP64: 8, U.FP32: 4, U.MIXED_BF16R: 4, U.MIXED_F64: 4}.get(prec, 2)   # coarse-level vector element size
     mode = {0: "plain", 1: "pmv", 2: "split"}[st.kernel_set]   # the level-0 kernel set the solves ran
     # the dominant kernel on the solve's critical path: in the split form the
     # x update runs on a second stream beside the coarse levels, so it is
@@ -613,7 +552,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "fp32", "fp64": "fp64", "bf16": "fp32+bf16", "bf16r": "fp32+bf16", "fp16": "fp32+fp16"}[args.precision],
+        "dtype": {"fp32": "fp32", "fp64": "fp64", "bf16": "fp32+bf16", "bf16r": "fp32+bf16", "mixed64": "fp64+fp32"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{pb.name}: 256^3 MAC pressure solve, dx=1/256, Neumann walls, floor slab y<0.1 + 8 seeded spheres, "
@@ -622,7 +561,7 @@ def main():
             "relres": 1e-6,
             "precision": {"fp32": "FP32 vectors+levels, FP64 reductions", "fp64": "FP64",
                           "bf16": "FP32 CG, BF16 levels", "bf16r": "FP32 CG and level vectors, BF16 level records",
-                          "fp16": "FP32 CG, FP16 levels"}[args.precision],
+                          "mixed64": "FP64 CG and level-0 smoothing, FP32 coarse levels"}[args.precision],
             "iters": k_med,
             "hbm_gbs_solve": round(hbm_gbs, 1),
             "alg_bytes_per_solve": alg / args.steps,

@@ -37,8 +37,11 @@ enum {
     UAAMG_FP64 = 0,            /* everything FP64: parity mode */
     UAAMG_FP32 = 1,            /* plain form: FP32 vectors and levels, FP64 reductions/scalars */
     UAAMG_MIXED_BF16 = 2,      /* FP32 CG; 3D coarse levels stored BF16 (FP32 arithmetic); 2D levels stay FP32 */
-    UAAMG_MIXED_FP16 = 3,      /* FP32 CG; coarse levels stored FP16 (h^2-normalised), FP32 arithmetic */
-    UAAMG_MIXED_BF16R = 4      /* FP32 CG and level vectors; 3D coarse-level stencil records stored BF16 */
+    UAAMG_MIXED_FP16 = 3,      /* reserved, not built: 1/h^2 couplings exceed the FP16 range at 256^3;
+                                  uaamg_create / uaamg_set_opts reject it with UAAMG_ERR_ARG */
+    UAAMG_MIXED_BF16R = 4,     /* FP32 CG and level vectors; 3D coarse-level stencil records stored BF16 */
+    UAAMG_MIXED_F64 = 5        /* FP64 CG vectors, scalars and level-0 smoothing; coarse levels FP32
+                                  (CudaPCG3D::solve_mixed, cuda_pcg_3d_opt.cu:319-330) */
 };
 
 enum { UAAMG_WALL_NEUMANN = 0, UAAMG_WALL_DIRICHLET = 1 };
@@ -66,7 +69,43 @@ typedef struct {
     int use_graph;             /* 1: replay queued iterations from a captured hipGraph */
     int timing;                /* bitmask of UAAMG_K_* kernel classes timed with HIP events (0 off, 0xff all) */
     int flexible;              /* 0: beta = (r,z)/rsold (pcg_3d.cpp:115); 1: Polak-Ribiere (flexible CG) */
+    /* Kernel forms and launch shapes.  Every form computes the same
+     * arithmetic; the defaults (uaamg_default_opts) are the fastest measured
+     * at 256^3.  Changing a structural form (UAAMG_FORM_ROWS, _ROWS_COARSE,
+     * _TAIL, _PMV) or rows_kc / march_* through uaamg_set_opts drops the grid
+     * (set it again). */
+    int forms;                 /* bitmask of UAAMG_FORM_* allowed (default UAAMG_FORM_DEFAULT) */
+    int xfork_level;           /* side-stream x update forked after this level's down kernel
+                                  (>= the tail's first level: right before the tail); default 2 */
+    int xblocks, xthreads;     /* x update launch: at most xblocks workgroups of xthreads (default 256 x 256) */
+    int rows_kc;               /* planes per level-0 row-kernel tile, multiple of 4 (0: auto) */
+    int march_kc_min;          /* fewest planes per march-kernel chunk, even (default 2) */
+    int march_target;          /* workgroups a march-kernel launch aims at (default 1024) */
+    int trace;                 /* 1: relative residual history of each solve to stderr */
+    int preconditioner;        /* UAAMG_PRECOND_UAAMG (key "pcg_uaamg", factory.cpp:23) or
+                                  UAAMG_PRECOND_NONE (key "cg": IdentityPreconditioner, z = r,
+                                  factory.cpp:19, factory_3d.cpp:19); changing it drops the grid */
 } uaamg_opts;
+
+enum { UAAMG_PRECOND_UAAMG = 0, UAAMG_PRECOND_NONE = 1 };
+
+/* kernel forms (uaamg_opts.forms, uaamg_stats.forms_used) */
+enum {
+    UAAMG_FORM_ROWS = 1 << 0,        /* level-0 row kernels (3D, FP32 vectors, nx <= 256) */
+    UAAMG_FORM_ROWS_COARSE = 1 << 1, /* record row kernels on levels with 64 < nx <= 128 */
+    UAAMG_FORM_TAIL = 1 << 2,        /* small coarse levels in one LDS-resident workgroup */
+    UAAMG_FORM_PMV = 1 << 3,         /* p update fused into the operator pass; with the fused
+                                        V-cycle also r update in the level-0 down kernel and
+                                        x update on a second stream */
+    UAAMG_FORM_XMASK = 1 << 4,       /* x update reads the mask (default: it does not need to) */
+    UAAMG_FORM_NT_X = 1 << 5,        /* nontemporal loads/stores in the x update */
+    UAAMG_FORM_NT_P = 1 << 6,        /* nontemporal loads of z, p_old in the operator pass */
+    UAAMG_FORM_NT_D = 1 << 7,        /* nontemporal loads of r_old, Ap in the level-0 down kernel */
+    UAAMG_FORM_NT_U = 1 << 8,        /* nontemporal loads of r in the level-0 up kernel */
+    UAAMG_FORM_SPLIT = 1 << 9,       /* (report only) the split CG update ran */
+    UAAMG_FORM_DEFAULT = UAAMG_FORM_ROWS | UAAMG_FORM_ROWS_COARSE | UAAMG_FORM_TAIL | UAAMG_FORM_PMV |
+                         UAAMG_FORM_NT_X | UAAMG_FORM_NT_P | UAAMG_FORM_NT_D | UAAMG_FORM_NT_U
+};
 
 typedef struct {
     int iters;                 /* CG loop iterations (matvecs) performed */
@@ -82,6 +121,7 @@ typedef struct {
     int    kernel_set;         /* level-0 CG kernels used: 0 separate passes; 1 p update fused into the
                                   operator; 2 in addition r update in the level-0 down kernel and the
                                   x update on a second stream, beside the coarse levels */
+    int    forms_used;         /* UAAMG_FORM_* bits of the kernel forms that ran */
 } uaamg_stats;
 
 /* kernel classes for uaamg_stats.ms_kernel */
@@ -110,6 +150,20 @@ int uaamg_set_opts(uaamg_ctx* ctx, const uaamg_opts* o);
 int uaamg_set_grid(uaamg_ctx* ctx, const uaamg_grid* g);
 /* Same, g->solid is a device pointer. */
 int uaamg_set_grid_device(uaamg_ctx* ctx, const uaamg_grid* g);
+
+/* Variable Laplacian of the current 2D grid: replaces Grid::lap_diag,
+ * lap_off_x, lap_off_y (include/core/grid.h:21-23, allocated by
+ * Grid::init_variable_lap, src/core/grid.cpp:12-17, filled for instance by
+ * SmoothCylinder::applyToLaplacian, src/geometry/cylinder_model.cpp:50-71),
+ * which PCG::solve's matvec reads when present (src/solver/pcg.cpp:36-46):
+ * Av = lap_diag v + lap_off_x (vL + vR) + lap_off_y (vB + vT) on fluid
+ * cells, a wall or solid neighbour taking the cell's own value.  The V-cycle
+ * keeps the uniform coefficients, as the reference's preconditioner does.
+ * Host FP64 padded arrays, all three or all NULL (back to the uniform
+ * operator); 2D grids only (UAAMG_ERR_ARG otherwise).  Kept until the next
+ * uaamg_set_grid, which drops them. */
+int uaamg_set_variable_lap(uaamg_ctx* ctx, const double* lap_diag, const double* lap_off_x,
+                           const double* lap_off_y);
 
 /* Replaces Solver::solve(Grid&, rhs, max_iter, tol) for key "pcg_uaamg"
  * (include/solver/solver.h:29-30, src/solver/pcg.cpp:18-126,
@@ -224,7 +278,9 @@ int uaamg_server_register(uaamg_server* s, int sim, const uaamg_grid* g);
  * reports the request done. */
 int uaamg_server_submit(uaamg_server* s, int sim, const double* rhs_dev, double* p_dev, int warm_start,
                         int max_iter, double tol, int* req);
-/* Admit, run one chunk, complete: done[0..*n_done) = finished request ids. */
+/* Admit, run one chunk, complete: done[0..*n_done) = finished request ids.
+ * Requests that finish while done[] is full are returned first by the next
+ * step (their latency is taken when they finish). */
 int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* n_done);
 /* Stats of a finished request (ms_total = latency from submit to completion). */
 int uaamg_server_request(const uaamg_server* s, int req, uaamg_stats* st);

@@ -63,8 +63,16 @@ class Grid : public Mesh2D {
 public:
     std::vector<double> u, v, p;
     std::vector<bool> solid;
+    // Variable Laplacian coefficients (grid.h:21-23); empty: the uniform operator
+    std::vector<double> lap_diag, lap_off_x, lap_off_y;
     Grid(int nx_, int ny_, double lx, double ly)
         : Mesh2D(nx_, ny_, lx, ly), u(u_size(), 0.0), v(v_size(), 0.0), p(p_size(), 0.0), solid(p_size(), false) {}
+    bool has_variable_lap() const { return !lap_diag.empty(); }
+    void init_variable_lap() {   // grid.cpp:12-17
+        lap_diag.assign(p_size(), 0.0);
+        lap_off_x.assign(p_size(), 0.0);
+        lap_off_y.assign(p_size(), 0.0);
+    }
     bool is_solid(int i, int j) const { return solid[ip(i, j)]; }
     void set_solid(int i, int j) { solid[ip(i, j)] = true; }
     double& u_at(int i, int j) { return u[iu(i, j)]; }
@@ -145,16 +153,21 @@ inline void check(uaamg_ctx* h, int rc, const char* what) {
 
 // PCG with the UAAMG preconditioner on the GPU, behind Solver.  opts default
 // to the reference (FP64, V(1,1), absolute tolerance); pass UAAMG_FP32 etc.
+// HipPCGSolver(false) is plain CG (the identity preconditioner), as
+// CudaPCGSolver(use_precond) (cuda_pcg_solver.h:16) and factory key "cg".
+// A Grid with a variable Laplacian (has_variable_lap) is solved with it, as
+// PCG::solve does (pcg.cpp:36-46).
 class HipPCGSolver : public Solver {
 public:
     explicit HipPCGSolver(const uaamg_opts* o = nullptr) {
         uaamg_default_opts(&opts_);
         if (o) opts_ = *o;
     }
+    explicit HipPCGSolver(bool use_precond, const uaamg_opts* o = nullptr) : HipPCGSolver(o) {
+        opts_.preconditioner = use_precond ? UAAMG_PRECOND_UAAMG : UAAMG_PRECOND_NONE;
+    }
     void solve(Grid& g, const std::vector<double>& rhs, int max_iter, double tol) override {
         if ((int)rhs.size() != g.p_size()) throw std::invalid_argument("rhs size != (nx+2)(ny+2)");
-        if (!ctx_) ctx_ = std::make_unique<detail::Ctx>(opts_);
-        mask_.assign(g.solid.begin(), g.solid.end());
         set_grid(g);
         uaamg_stats st{};
         int rc = uaamg_solve(ctx_->h, rhs.data(), g.p.data(), max_iter, tol, &st);
@@ -169,7 +182,9 @@ public:
         detail::check(ctx_->h, rc, "uaamg_project");
         last_ = st;
     }
-    std::string name() const override { return "PCG(HIP-UAAMG)"; }
+    std::string name() const override {
+        return opts_.preconditioner == UAAMG_PRECOND_NONE ? "CG(HIP)" : "PCG(HIP-UAAMG)";
+    }
     const uaamg_stats& last_stats() const { return last_; }
     int last_status() const { return last_.status; }
 
@@ -182,6 +197,13 @@ private:
         gd.h[0] = g.dx; gd.h[1] = g.dy; gd.h[2] = 1.0;
         gd.solid = mask_.data();
         detail::check(ctx_->h, uaamg_set_grid(ctx_->h, &gd), "uaamg_set_grid");
+        if (g.has_variable_lap()) {
+            const size_t n = (size_t)g.p_size();
+            if (g.lap_diag.size() != n || g.lap_off_x.size() != n || g.lap_off_y.size() != n)
+                throw std::invalid_argument("variable Laplacian arrays must hold (nx+2)(ny+2) values");
+            detail::check(ctx_->h, uaamg_set_variable_lap(ctx_->h, g.lap_diag.data(), g.lap_off_x.data(), g.lap_off_y.data()),
+                          "uaamg_set_variable_lap");
+        }
     }
     uaamg_opts opts_;
     std::unique_ptr<detail::Ctx> ctx_;
@@ -194,6 +216,9 @@ public:
     explicit HipPCGSolver3D(const uaamg_opts* o = nullptr) {
         uaamg_default_opts(&opts_);
         if (o) opts_ = *o;
+    }
+    explicit HipPCGSolver3D(bool use_precond, const uaamg_opts* o = nullptr) : HipPCGSolver3D(o) {
+        opts_.preconditioner = use_precond ? UAAMG_PRECOND_UAAMG : UAAMG_PRECOND_NONE;
     }
     void solve(Grid3D& g, const std::vector<double>& rhs, int max_iter, double tol) override {
         if ((int)rhs.size() != g.p_size()) throw std::invalid_argument("rhs size != (nx+2)(ny+2)(nz+2)");
@@ -211,7 +236,9 @@ public:
         detail::check(ctx_->h, rc, "uaamg_project");
         last_ = st;
     }
-    std::string name() const override { return "PCG3D(HIP-UAAMG3D)"; }
+    std::string name() const override {
+        return opts_.preconditioner == UAAMG_PRECOND_NONE ? "CG3D(HIP)" : "PCG3D(HIP-UAAMG3D)";
+    }
     const uaamg_stats& last_stats() const { return last_; }
     int last_status() const { return last_.status; }
 
@@ -231,23 +258,28 @@ private:
     uaamg_stats last_{};
 };
 
-// Factory::create of this library: "pcg_uaamg" on the HIP backend is the only
-// key with a back-end here; every other key (or the CPU backend) throws
-// instead of the reference's silent fallback to Jacobi (factory.cpp:24),
-// so a misspelled key cannot quietly select a different solver.
+// Factory::create of this library: "pcg_uaamg" (factory.cpp:23) and "cg"
+// (PCG with the identity preconditioner, factory.cpp:19, factory_3d.cpp:19)
+// on the HIP backend have back-ends here; every other key (or the CPU
+// backend) throws instead of the reference's silent fallback to Jacobi
+// (factory.cpp:24), so a misspelled key cannot quietly select a different
+// solver.  Factory3D maps "pcg_uaamg" to the UAAMG solve (the reference's
+// factory_3d.cpp:26-27 falls back to Jacobi3D for it).
 struct Factory {
     static std::unique_ptr<Solver> create(const std::string& key, Backend be = Backend::HIP,
                                           const uaamg_opts* o = nullptr) {
-        if (parse_solver_kind(key) == SolverKind::PCG_UAAMG && be == Backend::HIP)
-            return std::make_unique<HipPCGSolver>(o);
+        const SolverKind k = parse_solver_kind(key);
+        if (be == Backend::HIP && (k == SolverKind::PCG_UAAMG || k == SolverKind::CG))
+            return std::make_unique<HipPCGSolver>(k == SolverKind::PCG_UAAMG, o);
         throw std::invalid_argument("solver key '" + key + "' has no back-end in this library");
     }
 };
 struct Factory3D {
     static std::unique_ptr<Solver3D> create(const std::string& key, Backend be = Backend::HIP,
                                             const uaamg_opts* o = nullptr) {
-        if (parse_solver_kind(key) == SolverKind::PCG_UAAMG && be == Backend::HIP)
-            return std::make_unique<HipPCGSolver3D>(o);
+        const SolverKind k = parse_solver_kind(key);
+        if (be == Backend::HIP && (k == SolverKind::PCG_UAAMG || k == SolverKind::CG))
+            return std::make_unique<HipPCGSolver3D>(k == SolverKind::PCG_UAAMG, o);
         throw std::invalid_argument("solver key '" + key + "' has no back-end in this library");
     }
 };

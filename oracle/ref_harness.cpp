@@ -12,6 +12,9 @@
 #include "solver/pcg_3d.h"
 #include "solver/preconditioner/uaamg_preconditioner.h"
 #include "solver/preconditioner/uaamg_preconditioner_3d.h"
+#include "solver/preconditioner/identity_preconditioner.h"
+#include "solver/preconditioner/identity_preconditioner_3d.h"
+#include "geometry/cylinder_model.h"
 #include "pressure/pressure.h"
 #include "pressure/pressure_3d.h"
 #include <memory>
@@ -59,6 +62,46 @@ extern "C" int ref_pcg(int dim, int nx, int ny, int nz, double lx, double ly, do
     return 0;
 }
 
+// PCG with the preconditioner of a factory key (factory.cpp:19,23,
+// factory_3d.cpp:19): precond 0 = UAAMGPreconditioner[3D] ("pcg_uaamg"),
+// 1 = IdentityPreconditioner[3D] ("cg"); lap_diag/lap_off_x/lap_off_y
+// (2D, all or none) fill Grid's variable Laplacian (grid.h:21-23,
+// init_variable_lap, grid.cpp) that PCG::solve's matvec reads (pcg.cpp:36-46).
+extern "C" int ref_pcg_ex(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                          const unsigned char* solid, const double* rhs, double* p,
+                          int max_iter, double tol, int precond,
+                          const double* lap_diag, const double* lap_off_x, const double* lap_off_y) {
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        size_t N = g.p.size();
+        for (size_t c = 0; c < N; c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+        if (lap_diag) {
+            g.init_variable_lap();
+            for (size_t c = 0; c < N; c++) { g.lap_diag[c] = lap_diag[c]; g.lap_off_x[c] = lap_off_x[c]; g.lap_off_y[c] = lap_off_y[c]; }
+        }
+        std::vector<double> r(rhs, rhs + N);
+        std::unique_ptr<Preconditioner> pc;
+        if (precond == 1) pc = std::make_unique<IdentityPreconditioner>();
+        else pc = std::make_unique<UAAMGPreconditioner>();
+        PCG s(std::move(pc));
+        s.solve(g, r, max_iter, tol);
+        for (size_t c = 0; c < N; c++) p[c] = g.p[c];
+        return 0;
+    }
+    if (lap_diag) return -1;
+    Grid3D g(nx, ny, nz, lx, ly, lz);
+    size_t N = g.p.size();
+    for (size_t c = 0; c < N; c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }
+    std::vector<double> r(rhs, rhs + N);
+    std::unique_ptr<Preconditioner3D> pc;
+    if (precond == 1) pc = std::make_unique<IdentityPreconditioner3D>();
+    else pc = std::make_unique<UAAMGPreconditioner3D>();
+    PCG3D s(std::move(pc));
+    s.solve(g, r, max_iter, tol);
+    for (size_t c = 0; c < N; c++) p[c] = g.p[c];
+    return 0;
+}
+
 extern "C" int ref_uaamg_apply(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
                                const unsigned char* solid, const double* r, double* z) {
     if (dim == 2) {
@@ -76,6 +119,21 @@ extern "C" int ref_uaamg_apply(int dim, int nx, int ny, int nz, double lx, doubl
     std::vector<double> rv(r, r + N), zv(z, z + N);
     UAAMGPreconditioner3D pc; pc.apply(g, rv, zv);
     for (size_t c = 0; c < N; c++) z[c] = zv[c];
+    return 0;
+}
+
+// SmoothCylinder::applyToLaplacian (src/geometry/cylinder_model.cpp:50-71) on a
+// fresh Grid: the solid mask it marks and the variable Laplacian it writes.
+extern "C" int ref_smooth_cylinder_lap(int nx, int ny, double lx, double ly, double cx, double cy, double R,
+                                       double tw, unsigned char* solid, double* diag, double* off_x, double* off_y) {
+    Grid g(nx, ny, lx, ly);
+    g.init_variable_lap();
+    SmoothCylinder cyl(cx, cy, R, g.dx, g.dy, tw);
+    cyl.applyToLaplacian(g, g.lap_diag, g.lap_off_x, g.lap_off_y);
+    for (size_t c = 0; c < g.p.size(); c++) {
+        solid[c] = g.solid[c] ? 1 : 0;
+        diag[c] = g.lap_diag[c]; off_x[c] = g.lap_off_x[c]; off_y[c] = g.lap_off_y[c];
+    }
     return 0;
 }
 

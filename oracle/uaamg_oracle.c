@@ -55,6 +55,7 @@ void orc_default_opts(orc_opts* o) {
     o->coarsest_sweeps = 10;
     o->tol_mode = 0;
     o->project_mean = -1;
+    o->precond = ORC_PRECOND_UAAMG;
 }
 
 static inline size_t IX(const Level* L, int i, int j, int k) {
@@ -344,6 +345,8 @@ static int check_grid(const orc_grid* g) {
     if (!g || !g->solid) return -1;
     if (g->dim != 2 && g->dim != 3) return -1;
     if (g->nx < 1 || g->ny < 1 || (g->dim == 3 && g->nz < 1)) return -1;
+    const int nvar = (g->lap_diag != 0) + (g->lap_off_x != 0) + (g->lap_off_y != 0);
+    if (nvar != 0 && (nvar != 3 || g->dim != 2)) return -1;   /* all three, 2D only (grid.h:21-23) */
     return 0;
 }
 
@@ -443,6 +446,9 @@ static void matvec_u(const orc_grid* g, const double* v, double* Av) {
             double vF = (k > 1 && !S[GX(d, i, j, k - 1)]) ? v[GX(d, i, j, k - 1)] : ((k == 1 && w[4]) ? -vc : vc);
             double vK = (k < nz && !S[GX(d, i, j, k + 1)]) ? v[GX(d, i, j, k + 1)] : ((k == nz && w[5]) ? -vc : vc);
             Av[idx] = diag * vc - (vL + vR) * idx2 - (vB + vT) * idy2 - (vF + vK) * idz2;
+        } else if (g->lap_diag) {
+            /* variable coefficients, pcg.cpp:44: same neighbour rule, one off-diagonal per axis */
+            Av[idx] = g->lap_diag[idx] * vc + g->lap_off_x[idx] * (vL + vR) + g->lap_off_y[idx] * (vB + vT);
         } else {
             Av[idx] = diag * vc - (vL + vR) * idx2 - (vB + vT) * idy2;
         }
@@ -491,7 +497,9 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
     }
     memcpy(r, rhs, N * sizeof(double));
     if (project) subtract_mean(d, r);
-    apply_hier(H, g, r, z); rr.applies++;
+    const int ident = o->precond == ORC_PRECOND_IDENTITY;   /* z = r: identity_preconditioner.h:19 */
+    if (ident) memcpy(z, r, N * sizeof(double)); else apply_hier(H, g, r, z);
+    rr.applies++;
     if (project) subtract_mean(d, z);
     memcpy(pd, z, N * sizeof(double));
     double rsold = dotf(d, r, z);
@@ -513,7 +521,8 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
         rr.rnorm = sqrt(rsnew);
         int conv = (o->tol_mode == 1) ? (rsnew < tolsq) : (sqrt(rsnew) < tol);
         if (conv) { rr.status = 0; rr.iters = it + 1; goto out; }
-        apply_hier(H, g, r, z); rr.applies++;
+        if (ident) memcpy(z, r, N * sizeof(double)); else apply_hier(H, g, r, z);
+        rr.applies++;
         if (project) subtract_mean(d, z);
         double beta = dotf(d, r, z) / rsold;
         FOR_INT(d) { size_t id = GX(d, i, j, k); if (!d->s[id]) pd[id] = z[id] + beta * pd[id]; }

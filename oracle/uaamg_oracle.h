@@ -28,6 +28,12 @@ typedef struct {
     double dx, dy, dz;           /* spacing, L/n as mesh.h:21-22 */
     const unsigned char* solid;  /* padded mask, 1 = solid */
     int wall[6];                 /* -x,+x,-y,+y,-z,+z : ORC_NEUMANN (reference) or ORC_DIRICHLET */
+    /* Variable Laplacian of Grid (include/core/grid.h:21-23), 2D only: padded
+     * per-cell diagonal and x/y off-diagonal coefficients read by the PCG
+     * matvec (pcg.cpp:36-46).  All NULL: the uniform operator. */
+    const double* lap_diag;
+    const double* lap_off_x;
+    const double* lap_off_y;
 } orc_grid;
 
 typedef struct {
@@ -36,7 +42,12 @@ typedef struct {
     int coarsest_sweeps;         /* symmetric sweep pairs at the coarsest level (reference 10) */
     int tol_mode;                /* 0: sqrt(r.r) < tol (pcg.cpp:112); 1: r.r < tol^2 r0.r0 */
     int project_mean;            /* -1 auto (on iff every wall Neumann), 0 off, 1 on */
+    int precond;                 /* ORC_PRECOND_UAAMG (key "pcg_uaamg") or ORC_PRECOND_IDENTITY
+                                    (key "cg": IdentityPreconditioner, z = r; factory.cpp:19) */
 } orc_opts;
+
+#define ORC_PRECOND_UAAMG    0
+#define ORC_PRECOND_IDENTITY 1
 
 typedef struct {
     int iters;       /* loop iterations entered (matvecs); = CudaPCG3D::last_iters when converged */
@@ -68,8 +79,9 @@ int orc_level_data(const orc_grid* g, const orc_opts* o, int level,
                    unsigned char* solid, double* diag, double* cx, double* cy,
                    double* cz, double* excess);
 
-/* PCG matvec with uniform constants (pcg.cpp:47-59, pcg_3d.cpp:38-55),
- * extended with ghost = -v on Dirichlet walls. Writes interior cells. */
+/* PCG matvec with uniform constants (pcg.cpp:47-59, pcg_3d.cpp:38-55), or
+ * with the grid's variable coefficients (pcg.cpp:36-46) when set, extended
+ * with ghost = -v on Dirichlet walls. Writes interior cells. */
 int orc_matvec(const orc_grid* g, const double* v, double* Av);
 
 /* Standalone amgpcg (test/solver/test_amgpcg.cpp:6-70 with

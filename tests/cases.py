@@ -147,3 +147,18 @@ def mac_divergence(pb, u, v, w):
         W = w.reshape(nz + 1, ny + 2, nx + 2)
         d[1:-1, 1:-1, 1:-1] += (W[1:, 1:-1, 1:-1] - W[:-1, 1:-1, 1:-1]) / pb.h[2]
     return d.reshape(-1)
+
+
+def separable_lap(pb):
+    """Variable Laplacian coefficients (Grid::lap_diag / lap_off_x /
+    lap_off_y, grid.h:21-23) whose x couplings depend on y only and y
+    couplings on x only: with the matvec's one off-diagonal per axis
+    (pcg.cpp:44) the operator is symmetric with zero row sums
+    (diag = -2 off_x - 2 off_y), so PCG converges on it."""
+    import numpy as np
+    j, i = np.meshgrid(np.arange(pb.n[1] + 2), np.arange(pb.n[0] + 2), indexing="ij")
+    x, y = (i - 0.5) * pb.h[0], (j - 0.5) * pb.h[1]
+    ox = -(1.0 + 0.5 * np.sin(2 * np.pi * y / pb.L[1])) / pb.h[0] ** 2
+    oy = -(1.0 + 0.4 * np.cos(np.pi * x / pb.L[0])) / pb.h[1] ** 2
+    ox, oy = ox.reshape(-1), oy.reshape(-1)
+    return (-2 * ox - 2 * oy, ox, oy)

@@ -5,6 +5,7 @@
 //
 //   host_solver_test keys          key parsing / factory errors (no GPU)
 //   host_solver_test solve OUTDIR  GPU solves through the Solver interface
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -42,6 +43,14 @@ static int run_keys() {
     threw = false;
     try { Factory3D::create("pcg_uaamg", Backend::CPU); } catch (const std::invalid_argument&) { threw = true; }
     CHECK(threw);
+    threw = false;
+    try { Factory::create("pcg_gmg"); } catch (const std::invalid_argument&) { threw = true; }
+    CHECK(threw);
+    // "cg" and "pcg_uaamg" have HIP back-ends (contexts are created on the first solve)
+    CHECK(Factory::create("cg")->name() == "CG(HIP)");
+    CHECK(Factory3D::create("cg")->name() == "CG3D(HIP)");
+    CHECK(Factory::create("pcg_uaamg")->name() == "PCG(HIP-UAAMG)");
+    CHECK(Factory3D::create("pcg_uaamg")->name() == "PCG3D(HIP-UAAMG3D)");
     std::printf("keys %s\n", fails ? "FAILED" : "ok");
     return fails ? 1 : 0;
 }
@@ -95,6 +104,70 @@ static int run_solve(const std::string& out) {
         write_f64(out + "/g2_rhs.f64", rhs);
         write_u8(out + "/g2_solid.u8", g.solid);
         write_f64(out + "/g2_p1.f64", g.p);
+        // key "cg": the same system with the identity preconditioner
+        Grid gc(40, 24, 2.0, 1.2);
+        gc.solid = g.solid;
+        auto c = Factory::create("cg", Backend::HIP, &o);
+        c->solve(gc, rhs, 2000, 1e-10);
+        auto* hc = dynamic_cast<HipPCGSolver*>(c.get());
+        CHECK(hc && hc->last_status() == UAAMG_OK);
+        std::printf("2d cg iters %d status %d\n", hc->last_stats().iters, hc->last_status());
+        write_f64(out + "/g2_cg_p.f64", gc.p);
+        // variable Laplacian (grid.h:21-23): x couplings varying with y, y couplings with x
+        Grid gv(40, 24, 2.0, 1.2);
+        gv.solid = g.solid;
+        gv.init_variable_lap();
+        for (int j = 0; j < gv.ny + 2; ++j)
+            for (int i = 0; i < gv.nx + 2; ++i) {
+                const double ox = -(1.0 + 0.5 * std::sin(0.3 * j)) / (gv.dx * gv.dx);
+                const double oy = -(1.0 + 0.4 * std::cos(0.2 * i)) / (gv.dy * gv.dy);
+                gv.lap_off_x[gv.ip(i, j)] = ox;
+                gv.lap_off_y[gv.ip(i, j)] = oy;
+                gv.lap_diag[gv.ip(i, j)] = -2.0 * ox - 2.0 * oy;
+            }
+        auto v = Factory::create("pcg_uaamg", Backend::HIP, &o);
+        v->solve(gv, rhs, 500, 1e-10);
+        auto* hv = dynamic_cast<HipPCGSolver*>(v.get());
+        CHECK(hv && hv->last_status() == UAAMG_OK);
+        std::printf("2d variable-Laplacian iters %d status %d\n", hv->last_stats().iters, hv->last_status());
+        write_f64(out + "/g2v_diag.f64", gv.lap_diag);
+        write_f64(out + "/g2v_ox.f64", gv.lap_off_x);
+        write_f64(out + "/g2v_oy.f64", gv.lap_off_y);
+        write_f64(out + "/g2v_p.f64", gv.p);
+    }
+    // projection frames with a moving solid at fixed dims on one solver
+    // (the mask changes between frames; cuda_pcg_solver.cu:45-49 would keep the first)
+    {
+        Grid3D g(20, 16, 12, 1.25, 1.0, 0.75);
+        auto s = Factory3D::create("pcg_uaamg", Backend::HIP);
+        for (int f = 0; f < 3; ++f) {
+            std::fill(g.solid.begin(), g.solid.end(), false);
+            for (int k = 1; k <= g.nz; ++k)
+                for (int j = 1; j <= g.ny; ++j)
+                    for (int i = 1; i <= g.nx; ++i) {
+                        double dx = i - (7.0 + 3 * f), dy = j - 8.0, dz = k - 6.0;
+                        if (dx * dx + dy * dy + dz * dz < 7.0) g.set_solid(i, j, k);
+                    }
+            // the previous frame's projected velocity plus a frame-dependent perturbation
+            for (size_t q = 0; q < g.u.size(); ++q) g.u[q] += 0.5 * std::sin(0.37 * q + f);
+            for (size_t q = 0; q < g.v.size(); ++q) g.v[q] += 0.5 * std::cos(0.23 * q + 2 * f);
+            for (size_t q = 0; q < g.w.size(); ++q) g.w[q] += 0.5 * std::sin(0.11 * q - f);
+            const std::string pre = out + "/fr" + std::to_string(f);
+            write_f64(pre + "_u0.f64", g.u);
+            write_f64(pre + "_v0.f64", g.v);
+            write_f64(pre + "_w0.f64", g.w);
+            write_u8(pre + "_solid.u8", g.solid);
+            // the reference's options (FP64, absolute tolerance 1e-8): the solve ends at its
+            // p.Ap < 1e-15 breakdown return (pcg.cpp:100), as the reference's does
+            PressureProjection3D::project(g, 0.01, *s, 300, 1e-8);
+            auto* hs = dynamic_cast<HipPCGSolver3D*>(s.get());
+            CHECK(hs && hs->last_stats().iters > 5);
+            std::printf("frame %d projection iters %d status %d\n", f, hs->last_stats().iters, hs->last_status());
+            write_f64(pre + "_u.f64", g.u);
+            write_f64(pre + "_v.f64", g.v);
+            write_f64(pre + "_w.f64", g.w);
+            write_f64(pre + "_p.f64", g.p);
+        }
     }
     // projection, 3D: the grid of the first case with a smooth velocity field
     {

@@ -7,6 +7,9 @@ G1  2D UAAMG V-cycle z = M r   (uaamg_preconditioner.cpp:177-199)
 G2  3D UAAMG V-cycle           (uaamg_preconditioner_3d.cpp:198-229)
 G3  PCG p and preconditioner-application counts (pcg.cpp / pcg_3d.cpp)
 G4  standalone amgpcg x and iteration counts (test/solver/test_amgpcg.cpp)
+G5  PCG with the identity preconditioner (factory key "cg", factory.cpp:19)
+    and with the variable Laplacian of SmoothCylinder::applyToLaplacian
+    (cylinder_model.cpp:50-71) that PCG::solve's matvec reads (pcg.cpp:36-46)
 Inputs follow the reference's GPU tests where they exist
 (test_cuda_uaamg.cu:21-26, :70-75, :136-149; test_cuda_uaamg_3d.cu:15-25).
 """
@@ -18,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 import oracle_bind as ob  # noqa: E402
+import cases as shared_cases  # noqa: E402
 
 
 def grid_dims(pb):
@@ -104,9 +108,45 @@ def cases():
     return out
 
 
-def main():
+def g5_cases():
+    out = []
+    pb = ob.Problem((48, 40), (1.2, 1.0)); pb.solid = cyl_mask(pb, 0.4, 0.5, 0.15)
+    out.append(("g5_2d_48x40_cylinder_cg", pb, 1e3 * gaussian(pb), 400, 1e-6, ob.PRECOND_IDENTITY))
+    pb = ob.Problem((16, 8, 8), (1.0, 0.5, 0.5)); pb.solid = sphere_mask(pb, (0.6, 0.25, 0.25), 0.12)
+    out.append(("g5_3d_16x8x8_sphere_cg", pb, 1e3 * gaussian(pb), 200, 1e-6, ob.PRECOND_IDENTITY))
+    for prec, tag in ((ob.PRECOND_UAAMG, "uaamg"), (ob.PRECOND_IDENTITY, "cg")):
+        # symmetric coefficients (x couplings vary with y only, y couplings with x only): converges
+        pb = ob.Problem((48, 36), (1.2, 0.9)); pb.solid = cyl_mask(pb, 0.5, 0.45, 0.12)
+        pb.lap = shared_cases.separable_lap(pb)
+        out.append((f"g5_2d_48x36_separable_varlap_{tag}", pb, 1e3 * gaussian(pb), 600, 1e-6, prec))
+        # SmoothCylinder's coefficients give a non-symmetric operator whose range misses the
+        # zero-mean rhs: the reference's PCG does not converge; 40 iterations are the fixture
+        n, L = (64, 48), (1.0, 0.75)
+        solid, d, ox, oy = ob.ref_smooth_cylinder_lap(n, L, (0.5, 0.375), 0.3)
+        pb = ob.Problem(n, L, solid, lap=(d, ox, oy))
+        out.append((f"g5_2d_64x48_smoothcyl_varlap_{tag}_40it", pb, 1e3 * gaussian(pb), 40, 1e-12, prec))
+    return out
+
+
+def write_g5():
+    for name, pb, rhs, mi, rel, prec in g5_cases():
+        tol = rel * np.linalg.norm(rhs[pb.fluid_interior_mask()])    # absolute tolerance, relres rel
+        p = ob.ref_pcg_ex(pb, rhs, mi, tol, precond=prec)
+        d = dict(n=np.array(pb.n), L=np.array(pb.L), solid=pb.solid.copy(), rhs=rhs, p=p, max_iter=np.array(mi),
+                 tol=np.array(tol), precond=np.array(prec))
+        if pb.lap is not None:
+            d.update(lap_diag=pb.lap[0], lap_off_x=pb.lap[1], lap_off_y=pb.lap[2])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array("pcg_ex"), **d)
+        print("wrote", name)
+
+
+def main(groups):
     assert ob.ref_available(), "build oracle/_ref first: make -C oracle ref"
     os.makedirs(HERE, exist_ok=True)
+    if "g5" in groups:
+        write_g5()
+    if not ({"g1", "g2", "g3", "g4"} & set(groups)):
+        return
     index = []
     for name, kind, pb, arg in cases():
         d = dict(n=np.array(pb.n), L=np.array(pb.L), solid=pb.solid.copy())
@@ -139,4 +179,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # python make_golden.py [g1 g2 g3 g4 g5]  (default: all groups)
+    main(sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5"])

@@ -18,12 +18,17 @@ NEUMANN, DIRICHLET = 0, 1
 class OrcGrid(C.Structure):
     _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
-                ("solid", C.c_void_p), ("wall", C.c_int * 6)]
+                ("solid", C.c_void_p), ("wall", C.c_int * 6),
+                ("lap_diag", C.c_void_p), ("lap_off_x", C.c_void_p), ("lap_off_y", C.c_void_p)]
 
 
 class OrcOpts(C.Structure):
     _fields_ = [("nu_pre", C.c_int), ("nu_post", C.c_int), ("coarsest_side", C.c_int),
-                ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int), ("project_mean", C.c_int)]
+                ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int), ("project_mean", C.c_int),
+                ("precond", C.c_int)]
+
+
+PRECOND_UAAMG, PRECOND_IDENTITY = 0, 1
 
 
 class OrcResult(C.Structure):
@@ -69,6 +74,8 @@ def ref():
         _ref.ref_spacing.argtypes = [D, I]
         _ref.ref_spacing.restype = D
         _ref.ref_project.argtypes = [I, I, I, I, D, D, D, P, P, P, P, P, D, I, D]
+        _ref.ref_pcg_ex.argtypes = [I, I, I, I, D, D, D, P, P, P, I, D, I, P, P, P]
+        _ref.ref_smooth_cylinder_lap.argtypes = [I, I, D, D, D, D, D, D, P, P, P, P]
     return _ref
 
 
@@ -79,7 +86,10 @@ def _p(a):
 class Problem:
     """A padded grid problem: dims n, lengths L (dx = L/n), solid mask, walls."""
 
-    def __init__(self, n, L=None, solid=None, wall=None):
+    def __init__(self, n, L=None, solid=None, wall=None, lap=None):
+        """lap: None (uniform operator) or (diag, off_x, off_y) padded arrays, 2D
+        (Grid::lap_diag / lap_off_x / lap_off_y, grid.h:21-23)."""
+        self.lap = None if lap is None else tuple(np.ascontiguousarray(a, dtype=np.float64) for a in lap)
         self.dim = len(n)
         self.n = tuple(int(v) for v in n)
         self.L = tuple(float(v) for v in (L if L is not None else [1.0] * self.dim))
@@ -101,7 +111,9 @@ class Problem:
         g.solid = _p(self.solid)
         for f in range(6):
             g.wall[f] = self.wall[f]
-        self._keep = self.solid
+        if self.lap is not None:
+            g.lap_diag, g.lap_off_x, g.lap_off_y = (_p(a) for a in self.lap)
+        self._keep = (self.solid, self.lap)
         return g
 
     def view(self, a):
@@ -208,6 +220,29 @@ def ref_pcg(pb, rhs, max_iter, tol, p0=None):
     ref().ref_pcg(pb.dim, pb.n[0], pb.n[1], nz, pb.L[0], pb.L[1], lz, _p(pb.solid), _p(rhs), _p(p),
                   max_iter, tol, C.byref(ap))
     return p, ap.value
+
+
+def ref_pcg_ex(pb, rhs, max_iter, tol, precond=PRECOND_UAAMG, p0=None):
+    """The reference's PCG with the preconditioner of a factory key and the
+    problem's variable Laplacian (absolute tolerance, the reference's rule)."""
+    p = np.zeros(pb.N) if p0 is None else np.array(p0, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    nz = pb.n[2] if pb.dim == 3 else 1
+    lz = pb.L[2] if pb.dim == 3 else 1.0
+    lap = pb.lap if pb.lap is not None else (None, None, None)
+    rc = ref().ref_pcg_ex(pb.dim, pb.n[0], pb.n[1], nz, pb.L[0], pb.L[1], lz, _p(pb.solid), _p(rhs), _p(p),
+                          max_iter, tol, precond, *[(_p(a) if a is not None else None) for a in lap])
+    assert rc == 0, rc
+    return p
+
+
+def ref_smooth_cylinder_lap(n, L, centre, R, tw=0.0):
+    """The reference's SmoothCylinder::applyToLaplacian (cylinder_model.cpp:50-71)
+    on a fresh Grid: returns (solid, diag, off_x, off_y)."""
+    N = (n[0] + 2) * (n[1] + 2)
+    out = [np.zeros(N, np.uint8)] + [np.zeros(N) for _ in range(3)]
+    ref().ref_smooth_cylinder_lap(n[0], n[1], L[0], L[1], centre[0], centre[1], R, tw, *[_p(a) for a in out])
+    return tuple(out)
 
 
 def ref_uaamg_apply(pb, r):

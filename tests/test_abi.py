@@ -52,10 +52,13 @@ int main(void) {
   F(uaamg_opts, precision) F(uaamg_opts, nu_pre) F(uaamg_opts, nu_post) F(uaamg_opts, coarsest_side)
   F(uaamg_opts, coarsest_sweeps) F(uaamg_opts, tol_mode) F(uaamg_opts, max_iter) F(uaamg_opts, tol)
   F(uaamg_opts, project_mean) F(uaamg_opts, fused) F(uaamg_opts, check_every) F(uaamg_opts, use_graph)
-  F(uaamg_opts, timing) F(uaamg_opts, flexible)
+  F(uaamg_opts, timing) F(uaamg_opts, flexible) F(uaamg_opts, forms) F(uaamg_opts, xfork_level)
+  F(uaamg_opts, xblocks) F(uaamg_opts, xthreads) F(uaamg_opts, rows_kc) F(uaamg_opts, march_kc_min)
+  F(uaamg_opts, march_target) F(uaamg_opts, trace) F(uaamg_opts, preconditioner)
+  printf("UAAMG_FORM_DEFAULT %d\n", (int)UAAMG_FORM_DEFAULT);
   F(uaamg_stats, iters) F(uaamg_stats, status) F(uaamg_stats, nlevels) F(uaamg_stats, rnorm)
   F(uaamg_stats, r0norm) F(uaamg_stats, ms_total) F(uaamg_stats, ms_setup) F(uaamg_stats, ms_kernel)
-  F(uaamg_stats, n_kernel) F(uaamg_stats, alg_bytes) F(uaamg_stats, kernel_set)
+  F(uaamg_stats, n_kernel) F(uaamg_stats, alg_bytes) F(uaamg_stats, kernel_set) F(uaamg_stats, forms_used)
   return 0;
 }
 """
@@ -74,6 +77,7 @@ def test_struct_layout_matches_ctypes(tmp_path):
         for fname, _ in T._fields_:
             cname = {"GridDesc": "uaamg_grid", "Opts": "uaamg_opts", "Stats": "uaamg_stats"}[T.__name__]
             assert int(c[f"{cname}.{fname}"]) == getattr(T, fname).offset, (cname, fname)
+    assert int(c["UAAMG_FORM_DEFAULT"]) == U.FORM_DEFAULT
 
 
 def test_default_opts_reproduce_reference():
@@ -83,6 +87,7 @@ def test_default_opts_reproduce_reference():
     assert o.coarsest_side == 4 and o.coarsest_sweeps == 10    # uaamg_preconditioner.cpp:20, 147-150
     assert o.tol_mode == U.TOL_ABS                             # pcg.cpp:112
     assert o.project_mean == -1 and o.flexible == 0
+    assert o.forms == U.FORM_DEFAULT and o.trace == 0
 
 
 def test_no_device_is_an_error_not_a_fallback():
@@ -115,6 +120,45 @@ def test_bad_arguments_rejected():
     L = U.load().L
     assert L.uaamg_default_opts(None) == U.ERR_ARG
     assert L.uaamg_create(None, None) == U.ERR_ARG
+
+
+@pytest.mark.parametrize("kw", [{"precision": U.MIXED_FP16}, {"precision": 6}, {"forms": 1 << 12},
+                                {"xthreads": 32}, {"xblocks": 0}, {"march_target": 0}, {"rows_kc": -4},
+                                {"tol_mode": 2}, {"nu_pre": -1}, {"preconditioner": 2}])
+def test_invalid_options_are_argument_errors(kw):
+    """Options are checked before the device: an unbuilt precision (FP16
+    levels) or an out-of-range form or launch shape is UAAMG_ERR_ARG, not a
+    HIP error or a silent substitute."""
+    L = U.load().L
+    h = C.c_void_p()
+    o = U.load().default_opts(**kw)
+    assert L.uaamg_create(C.byref(h), C.byref(o)) == U.ERR_ARG
+    b = C.c_void_p()
+    o = U.load().default_opts(**{"precision": U.FP32, **kw})   # a precision the batch takes, unless under test
+    assert L.uaamg_batch_create(C.byref(b), C.byref(o)) == U.ERR_ARG
+    s = C.c_void_p()
+    assert L.uaamg_server_create(C.byref(s), C.byref(o), 1 << 30, 4) == U.ERR_ARG
+
+
+@pytest.mark.parametrize("kw", [{"forms": U.FORM_DEFAULT & ~U.FORM_TAIL}, {"preconditioner": U.PRECOND_NONE}],
+                         ids=["no_tail", "identity"])
+def test_batch_and_server_need_uaamg_with_tail(kw):
+    L = U.load().L
+    o = U.load().default_opts(precision=U.FP32, **kw)
+    b = C.c_void_p()
+    assert L.uaamg_batch_create(C.byref(b), C.byref(o)) == U.ERR_ARG
+    s = C.c_void_p()
+    assert L.uaamg_server_create(C.byref(s), C.byref(o), 1 << 30, 4) == U.ERR_ARG
+
+
+def test_bytes_required_mixed_f64_between_fp32_and_fp64():
+    """FP64 CG vectors with FP32 coarse levels: more than FP32, less than FP64."""
+    L = U.load().L
+    s = np.zeros(66 * 66 * 66, np.uint8)
+    g = capi.make_grid((64, 64, 64), (1 / 64,) * 3, s)
+    b = {p: L.uaamg_bytes_required(C.byref(g), C.byref(U.load().default_opts(precision=p)))
+         for p in (U.FP64, U.MIXED_F64, U.FP32)}
+    assert b[U.FP32] < b[U.MIXED_F64] < b[U.FP64]
 
 
 def test_host_mirror_keys():

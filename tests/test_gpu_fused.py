@@ -2,8 +2,6 @@
 level-0 down kernel, x update on a second stream) against the separate-pass
 form of the same engine, including the stop paths (converged, max_iter,
 breakdown on a zero rhs, a solve that stops inside a captured graph chunk)."""
-import os
-
 import numpy as np
 import pytest
 
@@ -17,16 +15,9 @@ FUSED = [c for c in cases.parity_cases() if c[0] in NAMES]
 
 
 def _ctx(pb, pmv, **opts):
-    old = os.environ.get("UAAMG_PMV")
-    os.environ["UAAMG_PMV"] = "1" if pmv else "0"
-    try:
-        c = U.Context(**opts)
-        c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
-    finally:
-        if old is None:
-            del os.environ["UAAMG_PMV"]
-        else:
-            os.environ["UAAMG_PMV"] = old
+    forms = opts.pop("forms", U.FORM_DEFAULT)
+    c = U.Context(forms=forms if pmv else forms & ~U.FORM_PMV, **opts)
+    c.set_grid(pb.n, pb.h, pb.solid, pb.wall)
     return c
 
 
@@ -46,6 +37,7 @@ def test_fused_matches_separate_passes(gpu, name, pb, use_graph):
     p1, s1 = _solve(pb, True, rhs, 500, 1e-6, use_graph=use_graph, check_every=4)
     p0, s0 = _solve(pb, False, rhs, 500, 1e-6, use_graph=use_graph, check_every=4)
     assert s1.kernel_set == 2 and s0.kernel_set == 0
+    assert s1.forms_used & U.FORM_SPLIT and not s0.forms_used & (U.FORM_PMV | U.FORM_SPLIT)
     assert s1.status == U.OK and s0.status == U.OK
     assert abs(s1.iters - s0.iters) <= 1, (s1.iters, s0.iters)
     m = pb.fluid_interior_mask()
@@ -120,30 +112,25 @@ def test_fused_warm_start_and_repeat(gpu, case):
     assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
 
 
-@pytest.mark.parametrize("env", [{"UAAMG_XNB": "1"}, {"UAAMG_XFORK": "0", "UAAMG_XBLOCKS": "2048"},
-                                 {"UAAMG_XFORK": "99", "UAAMG_XTHREADS": "512"},
-                                 {"UAAMG_XNT": "0", "UAAMG_PNT": "0", "UAAMG_DNT": "0", "UAAMG_UNT": "0"}],
+@pytest.mark.parametrize("opts,used,unused",
+                         [({"forms": U.FORM_DEFAULT | U.FORM_XMASK}, U.FORM_XMASK, U.FORM_NT_X),
+                          ({"xfork_level": 0, "xblocks": 2048}, U.FORM_NT, 0),
+                          ({"xfork_level": 99, "xthreads": 512}, U.FORM_NT, 0),
+                          ({"forms": U.FORM_DEFAULT & ~U.FORM_NT}, 0, U.FORM_NT)],
                          ids=["mask", "fork0_2048", "fork_tail_512", "temporal"])
-def test_fused_x_update_forms(gpu, env):
+def test_fused_x_update_forms(gpu, opts, used, unused):
     """The side-stream x update (x += alpha p, elementwise) in its other forms
     (mask read, other fork points and launch shapes), and plain instead of
     nontemporal loads and stores in it and the level-0 kernels, give
-    bit-identical
-    pressure to the default form: only where and how the same FP64 updates
-    run changes."""
+    bit-identical pressure to the default form: only where and how the same
+    FP64 updates run changes.  uaamg_stats.forms_used shows that the form
+    under test is the one that ran."""
     name, pb = FUSED[L1REC]
     rhs = cases.gaussian_rhs(pb)
     p_def, s_def = _solve(pb, True, rhs, 500, 1e-6, use_graph=1, check_every=4)
-    old = {k: os.environ.get(k) for k in env}
-    os.environ.update(env)
-    try:
-        p_alt, s_alt = _solve(pb, True, rhs, 500, 1e-6, use_graph=1, check_every=4)
-    finally:
-        for k, v in old.items():
-            if v is None:
-                del os.environ[k]
-            else:
-                os.environ[k] = v
+    p_alt, s_alt = _solve(pb, True, rhs, 500, 1e-6, use_graph=1, check_every=4, **opts)
     assert s_def.kernel_set == s_alt.kernel_set == 2
+    assert s_def.forms_used & U.FORM_NT == U.FORM_NT, hex(s_def.forms_used)   # the default runs every nontemporal form
+    assert s_alt.forms_used & used == used and not s_alt.forms_used & unused, hex(s_alt.forms_used)
     assert s_def.status == s_alt.status == U.OK and s_def.iters == s_alt.iters
     np.testing.assert_array_equal(p_alt, p_def)

@@ -82,3 +82,45 @@ def test_host_mirror_projection_matches_oracle(host_run):
     vmax = max(np.abs(u0).max(), np.abs(v0).max(), np.abs(w0).max())
     for name, ref in (("pj_u.f64", uo), ("pj_v.f64", vo), ("pj_w.f64", wo)):
         assert np.abs(_rd(d, name) - ref).max() <= 1e-9 * vmax
+
+
+def test_host_mirror_cg_matches_oracle(host_run):
+    """Factory::create("cg"): PCG with the identity preconditioner
+    (factory.cpp:19) equals the oracle's plain CG."""
+    d, _ = host_run
+    pb = ob.Problem((40, 24), (2.0, 1.2), solid=_rd(d, "g2_solid.u8", np.uint8))
+    rhs = _rd(d, "g2_rhs.f64")
+    po, res = ob.pcg(pb, rhs, 2000, 1e-10, tol_mode=1, precond=ob.PRECOND_IDENTITY)
+    assert res.status == 0
+    assert _fluid_err(pb, _rd(d, "g2_cg_p.f64"), po) <= 1e-9
+
+
+def test_host_mirror_variable_lap_matches_oracle(host_run):
+    """A Grid with a variable Laplacian (init_variable_lap) solved through
+    the factory's pcg_uaamg equals the oracle's PCG on the same coefficients."""
+    d, _ = host_run
+    lap = (_rd(d, "g2v_diag.f64"), _rd(d, "g2v_ox.f64"), _rd(d, "g2v_oy.f64"))
+    pb = ob.Problem((40, 24), (2.0, 1.2), solid=_rd(d, "g2_solid.u8", np.uint8), lap=lap)
+    rhs = _rd(d, "g2_rhs.f64")
+    po, res = ob.pcg(pb, rhs, 500, 1e-10, tol_mode=1)
+    assert res.status == 0
+    assert _fluid_err(pb, _rd(d, "g2v_p.f64"), po) <= 1e-9
+
+
+def test_host_mirror_projection_frames_moving_solid(host_run):
+    """Three PressureProjection3D::project frames on one solver, the solid
+    ball moving between frames at fixed dims: every frame equals the
+    oracle's projection with that frame's mask (a stale mask would not)."""
+    d, _ = host_run
+    masks = []
+    for f in range(3):
+        solid = _rd(d, f"fr{f}_solid.u8", np.uint8)
+        masks.append(solid)
+        pb = ob.Problem((20, 16, 12), (1.25, 1.0, 0.75), solid=solid)
+        u0, v0, w0 = (_rd(d, f"fr{f}_{c}0.f64") for c in "uvw")
+        uo, vo, wo, po, res = ob.project(pb, u0, v0, w0, 0.01, 300, 1e-8)
+        assert np.abs(_rd(d, f"fr{f}_p.f64") - po).max() <= 1e-9 * np.abs(po).max(), f
+        vmax = max(np.abs(u0).max(), np.abs(v0).max(), np.abs(w0).max())
+        for c, ref in zip("uvw", (uo, vo, wo)):
+            assert np.abs(_rd(d, f"fr{f}_{c}.f64") - ref).max() <= 1e-9 * vmax, (f, c)
+    assert not np.array_equal(masks[0], masks[1]) and not np.array_equal(masks[1], masks[2])

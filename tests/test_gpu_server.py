@@ -122,3 +122,30 @@ def test_server_admission_under_budget(gpu):
     assert status["bytes"] <= budget
     assert status["resident"] < len(small)     # some hierarchies were evicted
     srv.close()
+
+
+def test_server_step_returns_every_completion(gpu):
+    """A step returns at most max_done finished requests; the rest come back
+    from the following steps, each with its latency taken when it finished."""
+    import ctypes
+    import torch
+    srv = U.Server(1 << 34, max_active=16, precision=U.FP32, tol_mode=U.TOL_REL, use_graph=1)
+    for k, (name, pb) in enumerate(SIMS):
+        srv.register(k, pb.n, pb.h, pb.solid, pb.wall)
+    keep = [(_dev(_rhs(pb, 3)), _dev(np.zeros(pb.N))) for _, pb in SIMS]
+    reqs = [srv.submit(k, r.data_ptr(), p.data_ptr(), 0, 500, 1e-6) for k, (r, p) in enumerate(keep)]
+    done = (ctypes.c_int * 1)()
+    n = ctypes.c_int(0)
+    got = []
+    t0 = time.time()
+    while len(got) < len(reqs):
+        assert srv.lib.L.uaamg_server_step(srv.h, done, 1, ctypes.byref(n)) == U.OK
+        assert n.value <= 1
+        got += [done[0]] if n.value else []
+        assert time.time() - t0 < 60, "server did not finish"
+    torch.cuda.synchronize()
+    assert sorted(got) == sorted(reqs)
+    for r in reqs:
+        st = srv.request(r)
+        assert st is not None and st.status == U.OK and st.ms_total > 0, r
+    srv.close()

@@ -72,11 +72,12 @@ def test_pressure2d_mac_fp32_vs_fp64(gpu):
 @pytest.mark.parametrize("n", [128, 256])
 def test_plume3d_fp32_vs_fp64(gpu, n):
     """The benchmark workload (and its 128^3 version): the plain FP32 form at
-    relres 1e-6 converges within 12 % of the FP64 path's iterations (measured:
-    120 vs 113 at 128^3, 109 vs 101 at 256^3, from the FP32 CG vectors) and
-    agrees with the FP64 pressure.  An FP32 cancellation in the operator or
-    the V-cycle stalls CG at these sizes long before the small parity grids
-    show it."""
+    relres 1e-6 converges within 8 % of the FP64 path's iterations (measured:
+    120 vs 113 at 128^3, 106 vs 101 at 256^3; FP32 rounding anywhere in the
+    vectors or the preconditioner costs these, FP64 CG vectors alone do not
+    buy them back) and agrees with the FP64 pressure.  An FP32 cancellation
+    in the operator or the V-cycle stalls CG at these sizes long before the
+    small parity grids show it."""
     pb = W.plume3d(seed=0, n=n)
     out = {}
     for prec in (U.FP64, U.FP32):
@@ -87,7 +88,8 @@ def test_plume3d_fp32_vs_fp64(gpu, n):
         assert st.status == 0, (prec, U.status_name(st.status), st.iters)
         out[prec] = (p, st.iters)
     it64, it32 = out[U.FP64][1], out[U.FP32][1]
-    assert it32 <= it64 + max(2, (12 * it64) // 100), (it32, it64)
+    print(f"plume3d {n}^3: fp32 {it32} iterations, fp64 {it64}", flush=True)
+    assert it32 <= it64 + max(2, (8 * it64) // 100), (it32, it64)
     m = pb.fluid_mask()
     a = out[U.FP32][0][m]
     b = out[U.FP64][0][m]

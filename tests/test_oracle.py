@@ -18,6 +18,7 @@ import pytest
 
 import oracle_bind as ob
 import cases
+from uaamg_amd import workloads as W
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
 
@@ -41,6 +42,12 @@ def test_oracle_matches_golden(path):
         p, res = ob.pcg(pb, d["rhs"], int(d["max_iter"]), float(d["tol"]))
         assert np.array_equal(p, d["p"])
         assert res.applies == int(d["applies"])
+    elif kind == "pcg_ex":
+        lap = (d["lap_diag"], d["lap_off_x"], d["lap_off_y"]) if "lap_diag" in d.files else None
+        pb = ob.Problem(tuple(int(v) for v in d["n"]), tuple(float(v) for v in d["L"]), d["solid"], lap=lap)
+        p, res = ob.pcg(pb, d["rhs"], int(d["max_iter"]), float(d["tol"]), precond=int(d["precond"]))
+        assert res.status == (1 if "_40it" in path else 0)
+        assert np.array_equal(p, d["p"])
     else:
         N = int(d["N"])
         b = d["b"] if "b" in d.files else None
@@ -340,3 +347,100 @@ def test_project_rhs_linear_field():
     fl = pb.fluid_interior_mask()
     np.testing.assert_allclose(rhs[fl], (a + b + c) / dt, rtol=1e-12)
     assert np.all(rhs[~fl] == 0)
+
+
+# ---------------------------------------------------------------- "cg" key and the variable Laplacian
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (no reference tree here)")
+@pytest.mark.parametrize("name,pb", cases.neumann_cases(), ids=[c[0] for c in cases.neumann_cases()])
+def test_oracle_cg_matches_reference_build(name, pb):
+    """Key "cg": PCG with IdentityPreconditioner[3D] (factory.cpp:19,
+    factory_3d.cpp:19), bit for bit against the reference build."""
+    rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(5))
+    p1, res = ob.pcg(pb, rhs, 300, 1e-3, precond=ob.PRECOND_IDENTITY)
+    p2 = ob.ref_pcg_ex(pb, rhs, 300, 1e-3, precond=ob.PRECOND_IDENTITY)
+    assert np.array_equal(p1, p2)
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (no reference tree here)")
+@pytest.mark.parametrize("precond", [ob.PRECOND_UAAMG, ob.PRECOND_IDENTITY], ids=["uaamg", "cg"])
+@pytest.mark.parametrize("coef", ["separable", "smooth_cylinder"])
+def test_oracle_variable_lap_matches_reference_build(precond, coef):
+    """The variable-coefficient matvec (pcg.cpp:36-46) inside PCG, bit for
+    bit against the reference build (the V-cycle stays the uniform one, as in
+    the reference).  Symmetric coefficients (cases.separable_lap) converge;
+    the coefficients SmoothCylinder writes make a non-symmetric operator whose
+    range misses the zero-mean rhs, so the reference's PCG does not converge
+    and 40 iterations are compared."""
+    if coef == "separable":
+        pb = ob.Problem((41, 30), (1.0, 0.8))
+        s = pb.view(pb.solid); s[10:16, 12:20] = 1
+        pb.lap = cases.separable_lap(pb)
+        max_iter, rel = 600, 1e-6
+    else:
+        solid, d, ox, oy = W.smooth_cylinder_lap((40, 32), (1.0, 0.8), (0.5, 0.4), 0.33)
+        pb = ob.Problem((40, 32), (1.0, 0.8), solid, lap=(d, ox, oy))
+        max_iter, rel = 40, 1e-12
+    rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(8))
+    tol = rel * np.linalg.norm(rhs[pb.fluid_interior_mask()])
+    p1, res = ob.pcg(pb, rhs, max_iter, tol, precond=precond)
+    p2 = ob.ref_pcg_ex(pb, rhs, max_iter, tol, precond=precond)
+    assert res.iters > 3 and res.status == (0 if coef == "separable" else 1)
+    assert np.array_equal(p1, p2)
+
+
+def test_smooth_cylinder_coefficients_match_reference():
+    """workloads.smooth_cylinder_lap (the tests' generator of variable
+    coefficients) equals SmoothCylinder::applyToLaplacian bit for bit."""
+    if not ob.ref_available():
+        pytest.skip("oracle/_ref not built (no reference tree here)")
+    for n, L, c, R, tw in (((64, 48), (1.0, 0.75), (0.5, 0.37), 0.3, 0.0), ((33, 17), (2.0, 1.0), (1.1, 0.45), 0.4, 0.05)):
+        a = W.smooth_cylinder_lap(n, L, c, R, tw)
+        b = ob.ref_smooth_cylinder_lap(n, L, c, R, tw)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_variable_lap_with_uniform_coefficients_is_uniform():
+    """Coefficients diag = 2(1/dx^2 + 1/dy^2), off = -1/h^2 turn the
+    variable branch into the uniform operator exactly (a - b c == a + (-b) c
+    in IEEE arithmetic): matvec and PCG agree bit for bit with the uniform
+    form; a dropped or swapped coefficient would not."""
+    for name, pb in cases.neumann_cases():
+        if pb.dim != 2:
+            continue
+        ix, iy = 1 / pb.h[0] ** 2, 1 / pb.h[1] ** 2
+        var = ob.Problem(pb.n, pb.L, pb.solid, pb.wall,
+                         lap=(np.full(pb.N, 2 * (ix + iy)), np.full(pb.N, -ix), np.full(pb.N, -iy)))
+        v = cases.random_rhs(pb, np.random.default_rng(2))
+        assert np.array_equal(ob.matvec(var, v), ob.matvec(pb, v)), name
+        # swapping the axes' coefficients changes the operator on anisotropic grids
+        if ix != iy:
+            swp = ob.Problem(pb.n, pb.L, pb.solid, pb.wall,
+                             lap=(np.full(pb.N, 2 * (ix + iy)), np.full(pb.N, -iy), np.full(pb.N, -ix)))
+            assert not np.array_equal(ob.matvec(swp, v), ob.matvec(pb, v)), name
+        rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(3))
+        p1, r1 = ob.pcg(var, rhs, 200, 1e-4)
+        p2, r2 = ob.pcg(pb, rhs, 200, 1e-4)
+        assert np.array_equal(p1, p2) and r1.iters == r2.iters, name
+
+
+def test_cg_matches_dense_solve():
+    """Plain CG (identity preconditioner) converges to the dense solution of
+    the fluid-cell system (a least-squares solve of the singular Neumann
+    matrix, mean removed), on a grid with a solid block."""
+    pb = ob.Problem((9, 7), (1.0, 0.8))
+    s = pb.view(pb.solid); s[3:5, 4:6] = 1
+    m = pb.fluid_interior_mask()
+    idx = np.flatnonzero(m)
+    A = np.zeros((idx.size, idx.size))
+    for c, k in enumerate(idx):
+        e = np.zeros(pb.N); e[k] = 1.0
+        A[:, c] = ob.matvec(pb, e)[idx]
+    rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(4))
+    rhs[m] -= rhs[m].mean()
+    p, res = ob.pcg(pb, rhs, 500, 1e-8, precond=ob.PRECOND_IDENTITY)
+    assert res.status == 0
+    x = np.linalg.lstsq(A, -rhs[idx], rcond=None)[0]
+    a, b = p[idx] - p[idx].mean(), x - x.mean()
+    assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()

@@ -5,7 +5,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-FP64, FP32, MIXED_BF16, MIXED_FP16, MIXED_BF16R = 0, 1, 2, 3, 4
+FP64, FP32, MIXED_BF16, MIXED_FP16, MIXED_BF16R, MIXED_F64 = 0, 1, 2, 3, 4, 5
 TOL_ABS, TOL_REL = 0, 1
 NEUMANN, DIRICHLET = 0, 1
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NODEVICE, ERR_ADMISSION = 0, 1, 2, 3, 4, 5
@@ -13,6 +13,11 @@ NOT_CONVERGED, BREAKDOWN, NONFINITE = 10, 11, 12
 _NAMES = {0: "ok", 1: "bad-argument", 2: "hip-error", 3: "out-of-memory", 4: "no-device",
           5: "admission", 10: "not-converged", 11: "breakdown", 12: "nonfinite"}
 K_MATVEC, K_UPDATE, K_DOWN0, K_UP0, K_PUPDATE, K_COARSE, K_INIT, K_OTHER = range(8)
+FORM_ROWS, FORM_ROWS_COARSE, FORM_TAIL, FORM_PMV, FORM_XMASK = 1, 2, 4, 8, 16
+FORM_NT_X, FORM_NT_P, FORM_NT_D, FORM_NT_U, FORM_SPLIT = 32, 64, 128, 256, 512
+FORM_NT = FORM_NT_X | FORM_NT_P | FORM_NT_D | FORM_NT_U
+PRECOND_UAAMG, PRECOND_NONE = 0, 1
+FORM_DEFAULT = FORM_ROWS | FORM_ROWS_COARSE | FORM_TAIL | FORM_PMV | FORM_NT
 
 
 def status_name(rc):
@@ -29,18 +34,20 @@ class Opts(C.Structure):
                 ("coarsest_side", C.c_int), ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int),
                 ("max_iter", C.c_int), ("tol", C.c_double), ("project_mean", C.c_int),
                 ("fused", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
-                ("timing", C.c_int), ("flexible", C.c_int)]
+                ("timing", C.c_int), ("flexible", C.c_int), ("forms", C.c_int), ("xfork_level", C.c_int),
+                ("xblocks", C.c_int), ("xthreads", C.c_int), ("rows_kc", C.c_int), ("march_kc_min", C.c_int),
+                ("march_target", C.c_int), ("trace", C.c_int), ("preconditioner", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("status", C.c_int), ("nlevels", C.c_int),
                 ("rnorm", C.c_double), ("r0norm", C.c_double), ("ms_total", C.c_double),
                 ("ms_setup", C.c_double), ("ms_kernel", C.c_double * 8), ("n_kernel", C.c_int * 8),
-                ("alg_bytes", C.c_double), ("kernel_set", C.c_int)]
+                ("alg_bytes", C.c_double), ("kernel_set", C.c_int), ("forms_used", C.c_int)]
 
 
 EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",
-           "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_solve", "uaamg_solve_device",
+           "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_set_variable_lap", "uaamg_solve", "uaamg_solve_device",
            "uaamg_apply_precond", "uaamg_matvec", "uaamg_hierarchy", "uaamg_level_data",
            "uaamg_amgpcg", "uaamg_bytes_required", "uaamg_project", "uaamg_project_device",
            "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
@@ -71,6 +78,7 @@ class Lib:
         L.uaamg_set_opts.argtypes = [P, P]
         L.uaamg_set_grid.argtypes = [P, P]
         L.uaamg_set_grid_device.argtypes = [P, P]
+        L.uaamg_set_variable_lap.argtypes = [P, P, P, P]
         L.uaamg_solve.argtypes = [P, P, P, I, D, P]
         L.uaamg_solve_device.argtypes = [P, P, P, I, I, D, P]
         L.uaamg_apply_precond.argtypes = [P, P, P]
@@ -114,7 +122,9 @@ class Lib:
 
     def amgpcg(self, N, b, max_iter, tol, precision=FP64):
         x = np.zeros((N + 2) * (N + 2))
-        b = np.ascontiguousarray(b, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        if b.size != x.size:
+            raise ValueError(f"b: {b.size} values, N = {N} needs {x.size}")
         it = C.c_int(0)
         rc = self.L.uaamg_amgpcg(N, _p(b), _p(x), max_iter, tol, C.byref(it), precision)
         if rc != OK:
@@ -187,15 +197,44 @@ class Context:
         self.dim = len(n)
         self._solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
         self.N = int(np.prod([v + 2 for v in n]))
-        assert self._solid.size == self.N, (self._solid.size, self.N)
+        if self._solid.size != self.N:
+            raise ValueError(f"solid: {self._solid.size} values, the grid needs {self.N}")
         g = make_grid(n, h, self._solid, wall)
         rc = self.lib.L.uaamg_set_grid(self.h, C.byref(g))
         if rc != OK:
             self._err("set_grid", rc)
 
+    def _cells(self, a, what, n=None):
+        """C-order FP64 copy of a padded array, checked against the grid's size."""
+        if self.n is None:
+            raise RuntimeError("no grid set")
+        n = self.N if n is None else n
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if a.size != n:
+            raise ValueError(f"{what}: {a.size} values, the grid needs {n}")
+        return a
+
+    def mac_sizes(self):
+        """Lengths of the MAC face arrays u, v, w (mesh.h:27-35, mesh_3d.h:29-37)."""
+        nx, ny = self.n[0], self.n[1]
+        if self.dim == 2:
+            return (nx + 1) * (ny + 2), (nx + 2) * (ny + 1), 0
+        nz = self.n[2]
+        return (nx + 1) * (ny + 2) * (nz + 2), (nx + 2) * (ny + 1) * (nz + 2), (nx + 2) * (ny + 2) * (nz + 1)
+
+    def set_variable_lap(self, lap):
+        """lap = (diag, off_x, off_y) padded arrays (2D), or None for the uniform operator."""
+        if lap is None:
+            rc = self.lib.L.uaamg_set_variable_lap(self.h, None, None, None)
+        else:
+            self._lap = [self._cells(a, name) for a, name in zip(lap, ("lap_diag", "lap_off_x", "lap_off_y"))]
+            rc = self.lib.L.uaamg_set_variable_lap(self.h, *[_p(a) for a in self._lap])
+        if rc != OK:
+            self._err("set_variable_lap", rc)
+
     def solve(self, rhs, max_iter=-1, tol=-1.0, p0=None):
-        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
-        p = np.zeros(self.N) if p0 is None else np.array(p0, dtype=np.float64)
+        rhs = self._cells(rhs, "rhs")
+        p = np.zeros(self.N) if p0 is None else self._cells(p0, "p0").copy()
         st = Stats()
         rc = self.lib.L.uaamg_solve(self.h, _p(rhs), _p(p), int(max_iter), float(tol), C.byref(st))
         if rc not in (OK, NOT_CONVERGED, BREAKDOWN, NONFINITE):
@@ -212,9 +251,12 @@ class Context:
 
     def project(self, u, v, w, dt, max_iter=-1, tol=-1.0):
         """Pressure projection on host MAC arrays (copies); returns (u, v, w, p, Stats)."""
-        u = np.array(u, dtype=np.float64)
-        v = np.array(v, dtype=np.float64)
-        w = np.array(w, dtype=np.float64) if w is not None else None
+        nu, nv, nw = self.mac_sizes()
+        u = self._cells(u, "u", nu).copy()
+        v = self._cells(v, "v", nv).copy()
+        if self.dim == 3 and w is None:
+            raise ValueError("w: a 3D projection needs w")
+        w = self._cells(w, "w", nw).copy() if self.dim == 3 else None
         p = np.zeros(self.N)
         st = Stats()
         rc = self.lib.L.uaamg_project(self.h, _p(u), _p(v), _p(w) if w is not None else None, _p(p), float(dt),
@@ -233,7 +275,7 @@ class Context:
         return st
 
     def apply(self, r):
-        r = np.ascontiguousarray(r, dtype=np.float64)
+        r = self._cells(r, "r")
         z = np.zeros(self.N)
         rc = self.lib.L.uaamg_apply_precond(self.h, _p(r), _p(z))
         if rc != OK:
@@ -241,7 +283,7 @@ class Context:
         return z
 
     def matvec(self, v):
-        v = np.ascontiguousarray(v, dtype=np.float64)
+        v = self._cells(v, "v")
         Av = np.zeros(self.N)
         d = C.c_double(0)
         rc = self.lib.L.uaamg_matvec(self.h, _p(v), _p(Av), C.byref(d))

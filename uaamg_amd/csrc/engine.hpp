@@ -104,6 +104,9 @@ inline void tlaunch(Timer& t, int cls, void (*kern)(KP...), dim3 grid, dim3 bloc
     }
 }
 
+// forms fixed when an engine is built (buffers and launch shapes depend on them)
+constexpr int kStructForms = UAAMG_FORM_ROWS | UAAMG_FORM_ROWS_COARSE | UAAMG_FORM_TAIL | UAAMG_FORM_PMV;
+
 class EngineBase {
 public:
     virtual ~EngineBase() = default;
@@ -118,6 +121,8 @@ public:
     virtual void set_opts(const uaamg_opts& o) = 0;
     virtual void set_stream(hipStream_t s) = 0;
     virtual const uint8_t* solid_dev() const = 0;   // the level-0 solid mask (padded, device)
+    // variable Laplacian (2D): padded FP64 device arrays, copied; all null: uniform operator
+    virtual void set_variable_lap(const double* d, const double* ox, const double* oy) = 0;
 };
 
 // Hierarchy shape, shared by engines and the byte estimator (buildHierarchy,
@@ -183,7 +188,8 @@ public:
         int all_neumann = 1;
         for (int f = 0; f < 2 * D; f++) if (wall[f] != UAAMG_WALL_NEUMANN) all_neumann = 0;
         project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
-        rows_setup(h, wall);
+        ident_ = o.preconditioner == UAAMG_PRECOND_NONE;
+        if (!ident_) rows_setup(h, wall);   // the identity form runs the flat kernels only
         timer_.s = stream_;
         if (pmv_on_) {   // the x update's stream, created before any capture
             HIPCHK(hipStreamCreateWithFlags(&xs_, hipStreamNonBlocking));
@@ -203,6 +209,10 @@ public:
         o_.tol_mode = o.tol_mode; o_.max_iter = o.max_iter; o_.tol = o.tol; o_.fused = o.fused;
         o_.check_every = o.check_every; o_.use_graph = o.use_graph; o_.timing = o.timing; o_.flexible = o.flexible;
         o_.project_mean = o.project_mean;
+        // launch-time forms only: the structural ones (kStructForms, rows_kc,
+        // march_*) were fixed at construction, a change rebuilds the engine
+        o_.forms = (o_.forms & kStructForms) | (o.forms & ~kStructForms);
+        o_.xfork_level = o.xfork_level; o_.xblocks = o.xblocks; o_.xthreads = o.xthreads; o_.trace = o.trace;
         project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
         destroy_graph();
     }
@@ -213,8 +223,21 @@ public:
     size_t bytes() const override { return bytes_; }
 
     // ---------------- set-up ----------------
+    void set_variable_lap(const double* d, const double* ox, const double* oy) override {
+        if (D != 2) throw HipError{hipErrorInvalidValue, "variable Laplacian: 2D grids only (grid.h:21-23)"};
+        destroy_graph();
+        if (!d) { vlap_ = VarLap{}; return; }
+        if (!vlap_buf_) vlap_buf_ = dalloc<double>(3 * (size_t)g0_.N);
+        const size_t n = (size_t)g0_.N * sizeof(double);
+        HIPCHK(hipMemcpyAsync(vlap_buf_, d, n, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(vlap_buf_ + g0_.N, ox, n, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(vlap_buf_ + 2 * g0_.N, oy, n, hipMemcpyDeviceToDevice, stream_));
+        vlap_.d = vlap_buf_; vlap_.ox = vlap_buf_ + g0_.N; vlap_.oy = vlap_buf_ + 2 * g0_.N;
+    }
+
     void set_grid(const uint8_t* solid_dev) override {
         const int nl = nlevels();
+        vlap_ = VarLap{};   // a new grid drops the variable Laplacian
         HIPCHK(hipMemcpyAsync(solid0_, solid_dev, g0_.N, hipMemcpyDeviceToDevice, stream_));
         k_nbr0<D><<<lc0_.grid, lc0_.block, 0, stream_>>>(solid0_, nbr0_, g0_, lc0_.kz, partials_);
         k_fin_count<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)lc0_.nblocks(), count_);
@@ -240,6 +263,12 @@ public:
     // ---------------- V-cycle ----------------
     // Level 0 right-hand side is the CG residual r, its solution is z.
     void vcycle(const T* r, T* z, const int* skip) {
+        if (ident_) {   // key "cg": z = r (identity_preconditioner.h:19)
+            zsums_fused_ = false;
+            k_copy<T><<<flat_blocks(2048), 256, 0, stream_>>>(r, z, N4_, skip);
+            HIPCHK(hipGetLastError());
+            return;
+        }
         zsums_fused_ = o_.fused && o_.nu_pre == 1 && o_.nu_post == 1 && fused_ok_;
         if (zsums_fused_) vcycle_fused(r, z, skip);
         else vcycle_pass(r, z, skip);
@@ -296,22 +325,14 @@ public:
     static constexpr int CMAX = 2048;
 
     struct MLaunch { dim3 grid; int KC; };
-    static MLaunch march_launch(const Geo& g, int RX, int RY, long target) {
+    static MLaunch march_launch(const Geo& g, int RX, int RY, long target, int kc_min = 2) {
         int TX = RX - 4, TY = (D == 3) ? RY - 4 : 1;
         unsigned ta = (unsigned)((g.nx + TX - 1) / TX), tb = (D == 3) ? (unsigned)((g.ny + TY - 1) / TY) : 1u;
         int nm = (D == 3) ? g.nz : g.ny;
         long tiles = (long)ta * tb;
-        // small levels are latency-bound: short chunks (down to 2 planes) give
-        // more workgroups and fewer sequential plane steps
-        static const int kc_min = [] {
-            const char* e = std::getenv("UAAMG_MARCH_KCMIN");
-            return e ? std::max(2, std::atoi(e) / 2 * 2) : 2;
-        }();
-        static const long tmul = [] {
-            const char* e = std::getenv("UAAMG_MARCH_TARGET_X");
-            return e ? std::max(1L, std::atol(e)) : 1L;
-        }();
-        target *= tmul;
+        // small levels are latency-bound: short chunks (down to kc_min planes)
+        // give more workgroups and fewer sequential plane steps
+        kc_min = std::max(2, kc_min / 2 * 2);
         long chunks = std::max<long>(1, target / std::max<long>(1, tiles));
         int KC = (int)((nm + chunks - 1) / chunks);
         KC = std::max(KC, kc_min);
@@ -321,9 +342,12 @@ public:
         m.grid = dim3(ta, tb, (unsigned)((nm + KC - 1) / KC));
         return m;
     }
+    MLaunch mlaunch(const Geo& g, int RX, int RY) const {
+        return march_launch(g, RX, RY, std::max(1, o_.march_target), o_.march_kc_min);
+    }
     // ---- batch membership: this grid's buffers and launch shapes ----
     bool batch_ready() const {
-        return std::is_same<T, float>::value && fused_ok_ && tail_first_ > 0 && o_.nu_pre == 1 && o_.nu_post == 1
+        return std::is_same<T, float>::value && !ident_ && fused_ok_ && tail_first_ > 0 && o_.nu_pre == 1 && o_.nu_post == 1
             && nlevels() <= BATCH_MAXL;
     }
     double alg_bytes_of(int iters) const { return alg_bytes(iters); }
@@ -384,7 +408,7 @@ public:
     }
     void tail_setup() {
         tail_first_ = 0;
-        if (const char* env = std::getenv("UAAMG_TAIL")) if (env[0] == '0') return;
+        if (!(o_.forms & UAAMG_FORM_TAIL)) return;
         const int nl = nlevels();
         tail_first_ = choose_tail(shape_);
         if (!tail_first_) return;
@@ -415,8 +439,7 @@ public:
     void rows_setup(const double h[3], const int wall[6]) {
         rows_cpl_ = 0;
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            const char* env = std::getenv("UAAMG_ROWS");
-            if (env && env[0] == '0') return;
+            if (!(o_.forms & UAAMG_FORM_ROWS)) return;
             if (nlevels() < 2) return;
             const int nx = n_[0];
             rows_cpl_ = nx <= 128 ? 2 : (nx <= 256 ? 4 : 0);
@@ -442,22 +465,22 @@ public:
                 rtab_.any_dirichlet |= on;
             }
             ptab_.any_dirichlet = rtab_.any_dirichlet;
-            rgeo_ = make_rowgeo(n_[1], n_[2]);
+            rgeo_ = make_rowgeo(n_[1], n_[2], ROWS_NW * ROWS_RW - 4, o_.rows_kc);
             rows_blocks_ = 8u * (unsigned)rgeo_.per_xcd;
-            const char* pe = std::getenv("UAAMG_PMV");
-            pmv_on_ = !(pe && pe[0] == '0');
-            pgeo_ = make_rowgeo(n_[1], n_[2], PMV_NW * PMV_RW - 2);
+            pmv_on_ = (o_.forms & UAAMG_FORM_PMV) != 0;
+            pgeo_ = make_rowgeo(n_[1], n_[2], PMV_NW * PMV_RW - 2, o_.rows_kc);
             pmv_blocks_ = 8u * (unsigned)pgeo_.per_xcd;
         }
     }
     // Tiles of the row kernels: TY rows by KC planes, KC a multiple of 4 chosen
     // so that about 256 tiles (one per CU) cover the level (KC <= 16; more
     // planes per tile amortise the 4 halo plane steps, more tiles fill the chip)
-    static RowGeo make_rowgeo(int ny, int nz, int TY = ROWS_NW * ROWS_RW - 4) {
+    // unless kc_force > 0 sets it
+    static RowGeo make_rowgeo(int ny, int nz, int TY = ROWS_NW * ROWS_RW - 4, int kc_force = 0) {
         RowGeo rg{};
         rg.nty = (ny + TY - 1) / TY;
         int KC = (int)std::min<long>(16, std::max<long>(4, (long)nz * rg.nty / 256 / 4 * 4));
-        if (const char* kc = std::getenv("UAAMG_ROWS_KC")) KC = std::max(4, std::atoi(kc) / 4 * 4);
+        if (kc_force > 0) KC = std::max(4, kc_force / 4 * 4);
         rg.KC = KC;
         rg.nkc = (nz + KC - 1) / KC;
         rg.ntiles = rg.nty * rg.nkc;
@@ -470,8 +493,7 @@ public:
     void rows_coarse_setup() {
         for (int l = 0; l < 32; l++) rcpl_[l] = 0;
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            const char* env = std::getenv("UAAMG_ROWS_COARSE");
-            if (env && env[0] == '0') return;
+            if (!(o_.forms & UAAMG_FORM_ROWS_COARSE)) return;
             const int lt = tail_first_ ? tail_first_ : nlevels() - 1;
             for (int l = 1; l < lt && l < 32; l++) {
                 const Geo& g = shape_[l];
@@ -505,9 +527,10 @@ public:
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
             if (fr_old_) {   // alpha from the operator's partials; r = r_old - alpha Ap written to r; r.r -> convergence
-                // nontemporal loads of r_old and Ap (last use of both; UAAMG_DNT=0:
-                // plain): 34.5 vs 33.5 solves/s at 256^3 in three paired runs
-                static const bool dnt = [] { const char* e = std::getenv("UAAMG_DNT"); return !(e && e[0] == '0'); }();
+                // nontemporal loads of r_old and Ap (last use of both; without
+                // UAAMG_FORM_NT_D plain): 34.5 vs 33.5 solves/s at 256^3 in three paired runs
+                const bool dnt = (o_.forms & UAAMG_FORM_NT_D) && rows_cpl_ == 4;
+                forms_used_ |= dnt ? UAAMG_FORM_NT_D : 0;
                 tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? (dnt ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S, true> : k_rows_down0r<4, ROWS_NW, ROWS_RW, S>)
                                                               : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
@@ -522,9 +545,10 @@ public:
     }
     void rows_up0(const T* r, Level& C, T* z, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
-            // nontemporal loads of r (UAAMG_UNT=0: plain): 34.6 vs 33.9 solves/s
-            // at 256^3 in three paired runs
-            static const bool unt = [] { const char* e = std::getenv("UAAMG_UNT"); return !(e && e[0] == '0'); }();
+            // nontemporal loads of r (without UAAMG_FORM_NT_U plain): 34.6 vs
+            // 33.9 solves/s at 256^3 in three paired runs
+            const bool unt = (o_.forms & UAAMG_FORM_NT_U) && rows_cpl_ == 4;
+            forms_used_ |= unt ? UAAMG_FORM_NT_U : 0;
             tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? (unt ? k_rows_up0<4, ROWS_NW, ROWS_RW, S, true> : k_rows_up0<4, ROWS_NW, ROWS_RW, S>)
                                                         : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
@@ -534,7 +558,7 @@ public:
     }
 
     long fused_max_blocks() const {
-        MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+        MLaunch m = mlaunch(g0_, RX0, RY0);
         return (long)m.grid.x * m.grid.y * m.grid.z;
     }
     bool fused_supported() const { return nlevels() >= 2; }
@@ -547,9 +571,9 @@ public:
             Level& C = lv_[l + 1];
             if (l == 0 && rows_cpl_) {
                 rows_down0(r, C, skip);
-                if (fr_old_ && xfork_level_ == 0) fork_x();
+                if (fr_old_ && xfork_level() == 0) fork_x();
             } else if (l == 0) {
-                MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+                MLaunch m = mlaunch(g0_, RX0, RY0);
                 tlaunch(timer_, UAAMG_K_DOWN0, k_down_m<D, true, RX0, RY0, PPT0, T, C4, S, C4>, m.grid, dim3(256), stream_,
                         r, A0, C.b, (const C4*)C.coef, C.g, m.KC, skip);
             } else if (rcpl_[l]) {
@@ -559,14 +583,14 @@ public:
                 TScope ts(timer_, UAAMG_K_COARSE);
                 Level& F = lv_[l];
                 FusedArgs<D, false, SC, C4> A{F.g, nullptr, F.coef, Consts<SC>{}};
-                MLaunch m = march_launch(F.g, RXC, RYC, 1024);
+                MLaunch m = mlaunch(F.g, RXC, RYC);
                 k_down_m<D, false, RXC, RYC, PPTC, S, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
                     F.b, A, C.b, C.coef, C.g, m.KC, skip);
             }
             HIPCHK(hipGetLastError());
-            if (fr_old_ && l > 0 && xfork_level_ == l) fork_x();
+            if (fr_old_ && l > 0 && xfork_level() == l) fork_x();
         }
-        if (fr_old_ && xpend_ == false && xfork_level_ >= lt) fork_x();   // before the tail / coarsest solve
+        if (fr_old_ && xpend_ == false && xfork_level() >= lt) fork_x();   // before the tail / coarsest solve
         if (tail_first_) {
             TScope ts(timer_, UAAMG_K_COARSE);
             tail_launch(skip);
@@ -588,7 +612,7 @@ public:
             if (l == 0 && rows_cpl_) {
                 rows_up0(r, C, z, skip);
             } else if (l == 0) {
-                MLaunch m = march_launch(g0_, RX0, RY0, 1024);
+                MLaunch m = mlaunch(g0_, RX0, RY0);
                 tlaunch(timer_, UAAMG_K_UP0, k_up_m<D, true, RX0, RY0, PPT0, true, T, C4, T, S>, m.grid, dim3(256), stream_,
                         r, A0, (const S*)C.x, C.g, z, m.KC, skip, partials_);
                 k_fin_zsums<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)(m.grid.x * m.grid.y * m.grid.z), ctrl_);
@@ -599,7 +623,7 @@ public:
                 TScope ts(timer_, UAAMG_K_COARSE);
                 Level& F = lv_[l];
                 FusedArgs<D, false, SC, C4> A{F.g, nullptr, F.coef, Consts<SC>{}};
-                MLaunch m = march_launch(F.g, RXC, RYC, 1024);
+                MLaunch m = mlaunch(F.g, RXC, RYC);
                 k_up_m<D, false, RXC, RYC, PPTC, false, S, C4, S, S><<<m.grid, 256, 0, stream_>>>(
                     F.b, A, C.x, C.g, F.x, m.KC, skip, nullptr);
             }
@@ -617,7 +641,7 @@ public:
         {
             TScope ts(timer_, UAAMG_K_INIT);
             double* hist = nullptr;
-            if (trace_) {
+            if (trace()) {
                 if (hist_n_ < max_iter) {
                     if (hist_) HIPCHK(hipFree(hist_));
                     HIPCHK(hipMalloc(&hist_, sizeof(double) * std::max(1, max_iter)));
@@ -642,7 +666,8 @@ public:
             k_init_sub<T><<<fb, 256, 0, stream_>>>(r_, nbr0_, N4_, partials_, ctrl_);
             k_fin_init_sub<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             if (warm) {
-                k_init_warm<D, T><<<fb, 256, 0, stream_>>>(r_, x, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+                if (vlap_.d) k_init_warm<D, T, true><<<fb, 256, 0, stream_>>>(r_, x, nbr0_, g0_, c0d_, N4_, partials_, ctrl_, vlap_);
+                else k_init_warm<D, T><<<fb, 256, 0, stream_>>>(r_, x, nbr0_, g0_, c0d_, N4_, partials_, ctrl_, vlap_);
                 k_fin_init_warm<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)fb, ctrl_);
             }
             HIPCHK(hipGetLastError());
@@ -670,6 +695,7 @@ public:
         timer_.mask = (unsigned)o_.timing;
         timer_.reset();
         T* x = x_is_p() ? reinterpret_cast<T*>(p_dev) : x_;
+        forms_used_ = 0;
         // the split update needs the fused V-cycle (its level-0 row kernels)
         fr_ = pmv_on_ && !o_.flexible && o_.fused && o_.nu_pre == 1 && o_.nu_post == 1 && fused_ok_;
         start(rhs_dev, p_dev, warm, max_iter, tol, pmv_on_);
@@ -702,7 +728,7 @@ public:
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
         auto t1 = std::chrono::steady_clock::now();
-        if (trace_ && hc.iter > 0) {
+        if (trace() && hc.iter > 0) {
             std::vector<double> h(hc.iter);
             HIPCHK(hipMemcpy(h.data(), hist_, sizeof(double) * hc.iter, hipMemcpyDeviceToHost));
             std::fprintf(stderr, "[uaamg] r0=%.6e", std::sqrt(hc.r0sq));
@@ -721,6 +747,7 @@ public:
             timer_.collect(st);
             st->alg_bytes = alg_bytes(hc.iter);
             st->kernel_set = fr_ ? 2 : (pmv_on_ ? 1 : 0);
+            st->forms_used = forms_used();
         }
         return st ? st->status : 0;
     }
@@ -747,6 +774,17 @@ public:
         return (int)std::max(1.0, std::min(1e6, std::floor(0.75 * rem / every)));
     }
 
+    // UAAMG_FORM_* bits of the forms the last solve ran
+    int forms_used() const {
+        int f = forms_used_ | (pmv_on_ ? UAAMG_FORM_PMV : 0) | (fr_ ? UAAMG_FORM_SPLIT : 0);
+        if (zsums_fused_) {
+            f |= (rows_cpl_ ? UAAMG_FORM_ROWS : 0) | (tail_first_ ? UAAMG_FORM_TAIL : 0);
+            const int lt = tail_first_ ? tail_first_ : nlevels() - 1;
+            for (int l = 1; l < lt && l < 32; l++) f |= rcpl_[l] ? UAAMG_FORM_ROWS_COARSE : 0;
+        }
+        return f;
+    }
+
     // persistent grid for the flat level-0 kernels: at most `cap` workgroups of 256
     unsigned flat_blocks(unsigned cap) const {
         long need = (N4_ / 4 + 255) / 256;
@@ -761,6 +799,7 @@ public:
     hipGraph_t graph_[2] = {nullptr, nullptr};
     hipGraphExec_t gexec_[2] = {nullptr, nullptr};
     int graph_chunk_[2] = {0, 0};
+    int graph_forms_[2] = {0, 0};
     void destroy_graph() {
         for (int k = 0; k < 2; k++) {
             if (gexec_[k]) (void)hipGraphExecDestroy(gexec_[k]);
@@ -776,6 +815,8 @@ public:
             if (!graph_valid_ || graph_chunk_[k] != chunk) {
                 if (gexec_[k]) { (void)hipGraphExecDestroy(gexec_[k]); gexec_[k] = nullptr; }
                 if (graph_[k]) { (void)hipGraphDestroy(graph_[k]); graph_[k] = nullptr; }
+                const int f0 = forms_used_;
+                forms_used_ = 0;
                 HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 for (int c = 0; c < chunk; c++) iteration(x, skip);
                 join_x();   // every fork joins the capturing stream before the capture ends
@@ -783,9 +824,12 @@ public:
                 HIPCHK(hipGraphInstantiate(&gexec_[k], graph_[k], nullptr, nullptr, 0));
                 graph_valid_ = true;
                 graph_chunk_[k] = chunk;
+                graph_forms_[k] = forms_used_;   // the forms the graph's kernels use
+                forms_used_ = f0;
                 pcur_ = k;   // capture does not run the iterations
             }
             HIPCHK(hipGraphLaunch(gexec_[k], stream_));
+            forms_used_ |= graph_forms_[k];
             if (pmv_on_ && (chunk & 1)) pcur_ ^= 1;
             return;
         }
@@ -804,9 +848,10 @@ public:
             // them writes its r.r partials while other workgroups still read
             // (with the split update it also finishes the level-0 up kernel's z sums)
             if constexpr (D == 3 && std::is_same<T, float>::value) {
-                // nontemporal loads of z and p_old (last use of both; UAAMG_PNT=0:
-                // plain): 33.6 vs 33.0 solves/s at 256^3 in three paired runs
-                static const bool pnt = [] { const char* e = std::getenv("UAAMG_PNT"); return !(e && e[0] == '0'); }();
+                // nontemporal loads of z and p_old (last use of both; without
+                // UAAMG_FORM_NT_P plain): 33.6 vs 33.0 solves/s at 256^3 in three paired runs
+                const bool pnt = (o_.forms & UAAMG_FORM_NT_P) && fr_ && rows_cpl_ == 4;
+                forms_used_ |= pnt ? UAAMG_FORM_NT_P : 0;
                 auto kern = fr_ ? (rows_cpl_ == 4 ? (pnt ? k_rows_pmv<4, PMV_NW, PMV_RW, true, true> : k_rows_pmv<4, PMV_NW, PMV_RW, true>)
                                                   : k_rows_pmv<2, PMV_NW, PMV_RW, true>)
                                 : (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, false> : k_rows_pmv<2, PMV_NW, PMV_RW, false>);
@@ -818,8 +863,8 @@ public:
             mb = pmv_blocks_;
             if (!fr_) k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_m_, (int)mb, ctrl_);
         } else {
-            tlaunch(timer_, UAAMG_K_MATVEC, k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
-                    (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+            tlaunch(timer_, UAAMG_K_MATVEC, vlap_.d ? k_matvec_dot<D, T, true> : k_matvec_dot<D, T>, dim3(fb), dim3(256), stream_,
+                    (const T*)p_, Ap_, (const uint8_t*)nbr0_, g0_, c0d_, N4_, partials_, ctrl_, vlap_);
             k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)mb, ctrl_);
         }
         if (fr_) {   // r update and convergence test inside the level-0 down kernel
@@ -872,7 +917,8 @@ public:
     double matvec(const double* v_dev, double* Av_dev) override {
         k_ctrl_init<<<1, 64, 0, stream_>>>(ctrl_, count_, 1, 0.0, 0, project_, 0, nullptr);
         cast_in_all(v_dev, p_);
-        k_matvec_dot<D, T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_);
+        if (vlap_.d) k_matvec_dot<D, T, true><<<flat_blocks(2048), 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_, vlap_);
+        else k_matvec_dot<D, T><<<flat_blocks(2048), 256, 0, stream_>>>(p_, Ap_, nbr0_, g0_, c0d_, N4_, partials_, ctrl_, vlap_);
         k_fin_matvec<<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)flat_blocks(2048), ctrl_);
         cast_out_all(Ap_, Av_dev);
         HIPCHK(hipMemcpyAsync(ctrl_host_, ctrl_, sizeof(Ctrl), hipMemcpyDeviceToHost, stream_));
@@ -946,7 +992,8 @@ private:
         double cg = 13 * sv + 2;
         double l0 = 3 * sv + 2 * sl + 2 + 2 * rho * sl;
         double coarse = (5 * sl + 2 + 2 * ch * sc + 2 * rho * sl) * (rho / (1 - rho));
-        double per_iter = cg + l0 + coarse;
+        double per_iter = ident_ ? cg + 2 * sv : cg + l0 + coarse;   // identity: z = r copy instead of the V-cycle
+        if (vlap_.d) per_iter += 3 * 8;                              // the FP64 coefficient arrays of the operator
         double setup = 3 * sv + 3 + (1 + ch * sc) * (rho / (1 - rho));
         double init = 48;
         return N0 * (setup + init + iters * per_iter);
@@ -983,33 +1030,17 @@ private:
     // joined before the next operator pass (or the end of a captured chunk /
     // of the solve).  Default priority, like stream_.
     hipStream_t xs_ = nullptr;
-    // Fork point: after the down kernel of this level, or (any value >= the
-    // tail level) right before the tail.  The update runs as one 256-thread
-    // workgroup per CU: it then takes longer but leaves the coarse kernels
-    // room.  Measured at 256^3 (solves/s): 2048 workgroups forked before the
-    // tail 31.5; 256 workgroups forked after level 2 32.4; 192 or 128
-    // workgroups 30.3; 256 x 512 or 1024 threads 31.8 / 30.4; forked after
-    // level 0 with 128 workgroups 26.7.  UAAMG_XFORK / UAAMG_XBLOCKS /
-    // UAAMG_XTHREADS override.
-    int xfork_level_ = [] {
-        const char* e = std::getenv("UAAMG_XFORK");
-        return e ? std::max(0, std::atoi(e)) : 2;
-    }();
-    unsigned xblocks_ = [] {
-        const char* e = std::getenv("UAAMG_XBLOCKS");
-        return e ? (unsigned)std::max(8, std::atoi(e)) : 256u;
-    }();
-    unsigned xthreads_ = [] {
-        const char* e = std::getenv("UAAMG_XTHREADS");
-        return e ? (unsigned)std::min(1024, std::max(64, std::atoi(e) / 64 * 64)) : 256u;
-    }();
-    // whether the update reads the mask (UAAMG_XNB=1); by default it does
-    // not: the operator pass writes p = 0 on every non-fluid cell and p's
-    // ghosts stay 0, so x + alpha p leaves those cells of x as they are
-    bool xnb_ = [] {
-        const char* e = std::getenv("UAAMG_XNB");
-        return e ? e[0] != '0' : false;
-    }();
+    // Fork point (opts.xfork_level): after the down kernel of that level, or
+    // (any value >= the tail level) right before the tail.  The update runs
+    // as one 256-thread workgroup per CU (opts.xblocks x opts.xthreads): it
+    // then takes longer but leaves the coarse kernels room.  Measured at
+    // 256^3 (solves/s): 2048 workgroups forked before the tail 31.5; 256
+    // workgroups forked after level 2 32.4; 192 or 128 workgroups 30.3;
+    // 256 x 512 or 1024 threads 31.8 / 30.4; forked after level 0 with 128
+    // workgroups 26.7.  The update reads no mask unless UAAMG_FORM_XMASK is
+    // set: the operator pass writes p = 0 on every non-fluid cell and p's
+    // ghosts stay 0, so x + alpha p leaves those cells of x as they are.
+    int xfork_level() const { return std::max(0, o_.xfork_level); }
     hipEvent_t xfork_ = nullptr, xjoin_ = nullptr;
     bool xpend_ = false;
     T* xfork_x_ = nullptr;
@@ -1017,14 +1048,17 @@ private:
     void fork_x() {
         HIPCHK(hipEventRecord(xfork_, stream_));
         HIPCHK(hipStreamWaitEvent(xs_, xfork_, 0));
-        const long xneed = (N4_ / 4 + xthreads_ - 1) / xthreads_;
-        const unsigned xb = (unsigned)std::max<long>(1, std::min<long>(xneed, xblocks_));
-        // nontemporal loads and stores (UAAMG_XNT=0: plain): x and p are used
-        // once here, and the kernels running beside the update keep their
-        // working set in the caches (33.3 vs 32.9 solves/s at 256^3)
-        static const bool xnt = [] { const char* e = std::getenv("UAAMG_XNT"); return !(e && e[0] == '0'); }();
-        tlaunch(timer_, UAAMG_K_UPDATE, xnb_ ? k_xupdate<T, true> : (xnt ? k_xupdate<T, false, true> : k_xupdate<T, false>), dim3(xb),
-                dim3(xthreads_), xs_, xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const Ctrl*)ctrl_);
+        const unsigned xthreads = (unsigned)std::min(1024, std::max(64, o_.xthreads / 64 * 64));
+        const long xneed = (N4_ / 4 + xthreads - 1) / xthreads;
+        const unsigned xb = (unsigned)std::max<long>(1, std::min<long>(xneed, std::max(8, o_.xblocks)));
+        // nontemporal loads and stores (without UAAMG_FORM_NT_X plain): x and p
+        // are used once here, and the kernels running beside the update keep
+        // their working set in the caches (33.3 vs 32.9 solves/s at 256^3)
+        const bool xnb = (o_.forms & UAAMG_FORM_XMASK) != 0;
+        const bool xnt = !xnb && (o_.forms & UAAMG_FORM_NT_X);
+        forms_used_ |= (xnb ? UAAMG_FORM_XMASK : 0) | (xnt ? UAAMG_FORM_NT_X : 0);
+        tlaunch(timer_, UAAMG_K_UPDATE, xnb ? k_xupdate<T, true> : (xnt ? k_xupdate<T, false, true> : k_xupdate<T, false>), dim3(xb),
+                dim3(xthreads), xs_, xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const Ctrl*)ctrl_);
         HIPCHK(hipEventRecord(xjoin_, xs_));
         xpend_ = true;
     }
@@ -1052,7 +1086,11 @@ private:
     bool zsums_fused_ = false;
     Timer timer_;
     std::vector<void*> allocs_;
-    bool trace_ = std::getenv("UAAMG_TRACE") != nullptr;   // relres history to stderr (cf. PCG_DBG, cuda_pcg_3d_opt.cu:384)
+    bool trace() const { return o_.trace != 0; }   // relres history to stderr (cf. PCG_DBG, cuda_pcg_3d_opt.cu:384)
+    int forms_used_ = 0;                           // UAAMG_FORM_* bits that ran in the current solve
+    bool ident_ = false;                           // identity preconditioner (key "cg")
+    VarLap vlap_{};                                // variable Laplacian (2D; d == nullptr: uniform)
+    double* vlap_buf_ = nullptr;
     double* hist_ = nullptr; int hist_n_ = 0;
 
     // level-0 vectors carry a zeroed guard margin on both sides: the 4-cell

@@ -212,11 +212,34 @@ __device__ __forceinline__ double apply_A(long id, uint8_t nb, double vc, double
     return apply_A_at<D>(i, j, k, nb, vc, vl, vr, vb, vt, vf, vk, g, c);
 }
 
+// Variable Laplacian (2D): Grid::lap_diag / lap_off_x / lap_off_y
+// (grid.h:21-23), padded FP64 per-cell arrays; d == nullptr: uniform operator.
+struct VarLap {
+    const double* d = nullptr;
+    const double* ox = nullptr;
+    const double* oy = nullptr;
+};
+
+// The variable-coefficient operator (pcg.cpp:36-46): the same neighbour rule
+// as the uniform one, one off-diagonal coefficient per axis,
+// Av = diag v + off_x (vL + vR) + off_y (vB + vT), in that order.
+__device__ __forceinline__ double apply_A_var(int i, int j, long id, uint8_t nb, double vc, double vl, double vr,
+                                              double vb, double vt, const Geo& g, const Consts<double>& c,
+                                              const VarLap& L) {
+    double vL = (nb & NB_XM) ? vl : ((c.any_dirichlet && i == 1 && c.wall[0]) ? -vc : vc);
+    double vR = (nb & NB_XP) ? vr : ((c.any_dirichlet && i == g.nx && c.wall[1]) ? -vc : vc);
+    double vB = (nb & NB_YM) ? vb : ((c.any_dirichlet && j == 1 && c.wall[2]) ? -vc : vc);
+    double vT = (nb & NB_YP) ? vt : ((c.any_dirichlet && j == g.ny && c.wall[3]) ? -vc : vc);
+    return L.d[id] * vc + L.ox[id] * (vL + vR) + L.oy[id] * (vB + vT);
+}
+
 // Operator on a 4-cell vector at c0 (any fluid cell in it is interior, so
-// every neighbour index below is inside the padded array).
-template <int D, typename T>
+// every neighbour index below is inside the padded array).  VAR (2D only):
+// the variable-coefficient operator.
+template <int D, typename T, bool VAR = false>
 __device__ __forceinline__ void apply_A4(const T* __restrict__ p, long c0, uint32_t w, const vec4<T>& pc,
-                                         const Geo& g, const Consts<double>& c, double (&out)[4]) {
+                                         const Geo& g, const Consts<double>& c, double (&out)[4],
+                                         const VarLap& L = VarLap{}) {
     const vec4<T> pb = *(const vec4u<T>*)(p + c0 - g.sx);
     const vec4<T> pt = *(const vec4u<T>*)(p + c0 + g.sx);
     vec4<T> pf = (T)0, pk = (T)0;
@@ -227,16 +250,25 @@ __device__ __forceinline__ void apply_A4(const T* __restrict__ p, long c0, uint3
         uint8_t nb = nbyte(w, q);
         double vl = (q == 0) ? left : (double)pc[q == 0 ? 0 : q - 1];
         double vr = (q == 3) ? right : (double)pc[q == 3 ? 3 : q + 1];
-        out[q] = (nb & NB_FLUID) ? apply_A<D>(c0 + q, nb, (double)pc[q], vl, vr, (double)pb[q], (double)pt[q],
-                                              (double)pf[q], (double)pk[q], g, c)
-                                 : 0.0;
+        if constexpr (VAR && D == 2) {
+            int i = 0, j = 0, k = 0;
+            if (c.any_dirichlet) coords<D>(g, c0 + q, i, j, k);
+            out[q] = (nb & NB_FLUID) ? apply_A_var(i, j, c0 + q, nb, (double)pc[q], vl, vr, (double)pb[q],
+                                                   (double)pt[q], g, c, L)
+                                     : 0.0;
+        } else {
+            out[q] = (nb & NB_FLUID) ? apply_A<D>(c0 + q, nb, (double)pc[q], vl, vr, (double)pb[q], (double)pt[q],
+                                                  (double)pf[q], (double)pk[q], g, c)
+                                     : 0.0;
+        }
     }
 }
 
 // Ap = A p; pAp over fluid; alpha = rsold / pAp.  Non-fluid cells get Ap = 0.
-template <int D, typename T>
+template <int D, typename T, bool VAR = false>
 __device__ __forceinline__ void d_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const uint8_t* __restrict__ nbr, Geo g,
-                             Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
+                             Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n,
+                             VarLap L = VarLap{}) {
     if (ctrl->done) return;
     double v[1] = {0.0};
     FLAT_LOOP_XCD(N4) {
@@ -246,7 +278,7 @@ __device__ __forceinline__ void d_matvec_dot(const T* __restrict__ p, T* __restr
         if (w & 0x01010101u) {
             vec4<T> pc = *(const vec4<T>*)(p + c0);
             double o[4];
-            apply_A4<D, T>(p, c0, w, pc, g, c, o);
+            apply_A4<D, T, VAR>(p, c0, w, pc, g, c, o, L);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 if (!(nbyte(w, q) & NB_FLUID)) continue;
@@ -259,10 +291,10 @@ __device__ __forceinline__ void d_matvec_dot(const T* __restrict__ p, T* __restr
     }
     block_store<1>(v, partials);
 }
-template <int D, typename T>
+template <int D, typename T, bool VAR = false>
 __global__ void k_matvec_dot(const T* __restrict__ p, T* __restrict__ Ap, const uint8_t* __restrict__ nbr, Geo g,
-                             Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
-    d_matvec_dot<D, T>(p, Ap, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x);
+                             Consts<double> c, long N4, double* partials, Ctrl* ctrl, VarLap L) {
+    d_matvec_dot<D, T, VAR>(p, Ap, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x, L);
 }
 __device__ __forceinline__ void d_fin_matvec(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     if (ctrl->done) { if (threadIdx.x == 0) ctrl->xalpha = 0.0; return; }
@@ -283,9 +315,10 @@ __global__ void k_fin_matvec(const double* partials, int nb, Ctrl* ctrl) {
 }
 
 // Warm start: r -= A x0 (after the rhs projection); r.r recomputed.
-template <int D, typename T>
+template <int D, typename T, bool VAR = false>
 __device__ __forceinline__ void d_init_warm(T* __restrict__ r, const T* __restrict__ x0, const uint8_t* __restrict__ nbr, Geo g,
-                            Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n) {
+                            Consts<double> c, long N4, double* partials, Ctrl* ctrl, int fv_b, int fv_n,
+                            VarLap L = VarLap{}) {
     double v[1] = {0.0};
     FLAT_LOOP(N4) {
         long c0 = 4 * v_;
@@ -294,7 +327,7 @@ __device__ __forceinline__ void d_init_warm(T* __restrict__ r, const T* __restri
         vec4<T> pc = *(const vec4<T>*)(x0 + c0);
         vec4<T> rv = *(const vec4<T>*)(r + c0);
         double o[4];
-        apply_A4<D, T>(x0, c0, w, pc, g, c, o);
+        apply_A4<D, T, VAR>(x0, c0, w, pc, g, c, o, L);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (!(nbyte(w, q) & NB_FLUID)) continue;
@@ -306,10 +339,10 @@ __device__ __forceinline__ void d_init_warm(T* __restrict__ r, const T* __restri
     }
     block_store<1>(v, partials);
 }
-template <int D, typename T>
+template <int D, typename T, bool VAR = false>
 __global__ void k_init_warm(T* __restrict__ r, const T* __restrict__ x0, const uint8_t* __restrict__ nbr, Geo g,
-                            Consts<double> c, long N4, double* partials, Ctrl* ctrl) {
-    d_init_warm<D, T>(r, x0, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x);
+                            Consts<double> c, long N4, double* partials, Ctrl* ctrl, VarLap L) {
+    d_init_warm<D, T, VAR>(r, x0, nbr, g, c, N4, partials, ctrl, blockIdx.x, gridDim.x, L);
 }
 __device__ __forceinline__ void d_fin_init_warm(const double* partials, int nbt, int base, int nb, Ctrl* ctrl) {
     double tot[1];

@@ -188,4 +188,14 @@ __global__ void k_fill(V* __restrict__ x, long n, V v, const int* skip) {
     for (; t < n; t += stride) x[t] = v;
 }
 
+// z = r: the identity preconditioner of factory key "cg"
+// (identity_preconditioner.h:19, IdentityPreconditioner::apply)
+template <typename V>
+__global__ void k_copy(const V* __restrict__ src, V* __restrict__ dst, long n, const int* skip) {
+    if (skip && *skip) return;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) dst[t] = src[t];
+}
+
 }  // namespace uaamg

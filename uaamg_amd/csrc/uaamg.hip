@@ -200,11 +200,28 @@ extern "C" int uaamg_default_opts(uaamg_opts* o) {
     o->use_graph = 0;
     o->timing = 0;
     o->flexible = 0;
+    o->forms = UAAMG_FORM_DEFAULT;
+    o->xfork_level = 2;
+    o->xblocks = 256;
+    o->xthreads = 256;
+    o->rows_kc = 0;
+    o->march_kc_min = 2;
+    o->march_target = 1024;
+    o->trace = 0;
+    o->preconditioner = UAAMG_PRECOND_UAAMG;
     return UAAMG_OK;
 }
 
 static bool opts_valid(const uaamg_opts* o, std::string& why) {
-    if (o->precision < UAAMG_FP64 || o->precision > UAAMG_MIXED_BF16R) { why = "precision"; return false; }
+    if (o->precision < UAAMG_FP64 || o->precision > UAAMG_MIXED_F64) { why = "precision"; return false; }
+    if (o->precision == UAAMG_MIXED_FP16) {
+        why = "precision UAAMG_MIXED_FP16 is not built (1/h^2 couplings exceed the FP16 range; UAAMG_MIXED_BF16 is the 16-bit form)";
+        return false;
+    }
+    if (o->forms & ~(UAAMG_FORM_DEFAULT | UAAMG_FORM_XMASK)) { why = "forms: unknown bits"; return false; }
+    if (o->xfork_level < 0 || o->xblocks < 1 || o->xthreads < 64 || o->xthreads > 1024) { why = "x update launch"; return false; }
+    if (o->rows_kc < 0 || o->march_kc_min < 0 || o->march_target < 1) { why = "rows_kc / march_*"; return false; }
+    if (o->preconditioner != UAAMG_PRECOND_UAAMG && o->preconditioner != UAAMG_PRECOND_NONE) { why = "preconditioner"; return false; }
     if (o->nu_pre < 0 || o->nu_post < 0 || o->nu_pre > 16 || o->nu_post > 16) { why = "nu_pre/nu_post"; return false; }
     if (o->coarsest_side < 1) { why = "coarsest_side"; return false; }
     if (o->coarsest_sweeps < 0) { why = "coarsest_sweeps"; return false; }
@@ -233,7 +250,7 @@ extern "C" int uaamg_create(uaamg_ctx** out, const uaamg_opts* o) {
     uaamg_default_opts(&d);
     if (!o) o = &d;
     std::string why;
-    if (!opts_valid(o, why)) return UAAMG_ERR_ARG;
+    if (!opts_valid(o, why)) { std::fprintf(stderr, "uaamg_create: invalid option: %s\n", why.c_str()); return UAAMG_ERR_ARG; }
     int rc = check_device(why);
     if (rc) { std::fprintf(stderr, "uaamg_create: %s\n", why.c_str()); return rc; }
     uaamg_ctx* c = new (std::nothrow) uaamg_ctx();
@@ -260,7 +277,10 @@ extern "C" int uaamg_set_opts(uaamg_ctx* c, const uaamg_opts* o) {
     if (!c || !o) return UAAMG_ERR_ARG;
     std::string why;
     if (!opts_valid(o, why)) return fail(c, UAAMG_ERR_ARG, "invalid option: " + why);
-    bool rebuild = (o->precision != c->o.precision) || (o->coarsest_side != c->o.coarsest_side);
+    bool rebuild = (o->precision != c->o.precision) || (o->coarsest_side != c->o.coarsest_side) ||
+                   ((o->forms ^ c->o.forms) & uaamg::kStructForms) || o->rows_kc != c->o.rows_kc ||
+                   o->march_kc_min != c->o.march_kc_min || o->march_target != c->o.march_target ||
+                   o->preconditioner != c->o.preconditioner;
     c->o = *o;
     if (rebuild) c->eng.reset();     // a grid must be set again
     else if (c->eng) c->eng->set_opts(*o);
@@ -298,7 +318,8 @@ static std::unique_ptr<EngineBase> make_engine(const uaamg_opts& o, const uaamg_
             // stagnate (measured on the batched mixed-size stream); 3D loses ~10 %
             case UAAMG_MIXED_BF16: return std::make_unique<Engine<2, float, float>>(o, n, h, g.wall, s);
             case UAAMG_MIXED_BF16R: return std::make_unique<Engine<2, float, float>>(o, n, h, g.wall, s);
-            default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
+            case UAAMG_MIXED_F64: return std::make_unique<Engine<2, double, float>>(o, n, h, g.wall, s);
+            default: throw HipError{hipErrorInvalidValue, "precision not built"};   // opts_valid rejects it first
         }
     }
     switch (o.precision) {
@@ -306,7 +327,8 @@ static std::unique_ptr<EngineBase> make_engine(const uaamg_opts& o, const uaamg_
         case UAAMG_FP32: return std::make_unique<Engine<3, float, float>>(o, n, h, g.wall, s);
         case UAAMG_MIXED_BF16: return std::make_unique<Engine<3, float, __bf16>>(o, n, h, g.wall, s);
         case UAAMG_MIXED_BF16R: return std::make_unique<Engine<3, float, float, __bf16>>(o, n, h, g.wall, s);
-        default: throw HipError{hipErrorInvalidValue, "FP16 level storage is not built (MIXED_BF16 is the 16-bit form)"};
+        case UAAMG_MIXED_F64: return std::make_unique<Engine<3, double, float>>(o, n, h, g.wall, s);
+        default: throw HipError{hipErrorInvalidValue, "precision not built"};   // opts_valid rejects it first
     }
 }
 
@@ -357,6 +379,31 @@ static int ensure_stage(uaamg_ctx* c) {
     HIPCHK(hipMalloc(&c->rhs_stage, N * sizeof(double)));
     HIPCHK(hipMalloc(&c->p_stage, N * sizeof(double)));
     c->stage_n = N;
+    return UAAMG_OK;
+}
+
+extern "C" int uaamg_set_variable_lap(uaamg_ctx* c, const double* lap_diag, const double* lap_off_x,
+                                      const double* lap_off_y) {
+    if (!c) return UAAMG_ERR_ARG;
+    if (!c->eng) return fail(c, UAAMG_ERR_ARG, "no grid set");
+    const int nset = (lap_diag != nullptr) + (lap_off_x != nullptr) + (lap_off_y != nullptr);
+    if (nset != 0 && nset != 3) return fail(c, UAAMG_ERR_ARG, "variable Laplacian: pass all three arrays or none");
+    if (c->dim != 2) return fail(c, UAAMG_ERR_ARG, "variable Laplacian: 2D grids only (grid.h:21-23)");
+    CABI_TRY(c, {
+        if (nset == 0) {
+            c->eng->set_variable_lap(nullptr, nullptr, nullptr);
+        } else {
+            const long N = padded_n(c);
+            double* buf = nullptr;
+            HIPCHK(hipMalloc(&buf, 3 * N * sizeof(double)));
+            struct Free { double* p; ~Free() { (void)hipFree(p); } } fr{buf};
+            HIPCHK(hipMemcpyAsync(buf, lap_diag, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(buf + N, lap_off_x, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(buf + 2 * N, lap_off_y, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            c->eng->set_variable_lap(buf, buf + N, buf + 2 * N);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    })
     return UAAMG_OK;
 }
 
@@ -542,7 +589,8 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     auto shape = hierarchy_shape(g->dim, g->n[0], g->n[1], g->dim == 3 ? g->n[2] : 1, o->coarsest_side);
     if (shape.empty()) return 0;
     // mirrors Engine::allocate (device bytes requested by a context)
-    const size_t sv = (o->precision == UAAMG_FP64) ? 8 : 4;
+    const bool v64 = o->precision == UAAMG_FP64 || o->precision == UAAMG_MIXED_F64;
+    const size_t sv = v64 ? 8 : 4;
     // coarse-level vectors (b, x, excess) and stencil records: 16-bit only in
     // 3D (2D hierarchies keep FP32 levels under the 16-bit forms)
     const bool d3 = g->dim == 3;
@@ -554,7 +602,8 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     const size_t margin = (size_t)((g0.sx + 8 + 63) / 64) * 64;
     // level-0 vectors r, z, p, Ap, x and the second p and r buffers of the
     // fused CG kernels (3D FP32 forms with rows of at most 256 cells)
-    const bool pmv = g->dim == 3 && o->precision != UAAMG_FP64 && g0.nx <= 256;
+    const bool pmv = g->dim == 3 && !v64 && g0.nx <= 256 && shape.size() >= 2 &&
+                     (o->forms & UAAMG_FORM_ROWS) && (o->forms & UAAMG_FORM_PMV);
     size_t bytes = N0 + N4 + kRowSlack + 8 + sizeof(uaamg::Ctrl) + (pmv ? 7 : 5) * (N4 + 2 * margin) * sv;
     if (pmv) bytes += (size_t)(8 * ((((g0.ny + 15) / 16) * ((g0.nz + 3) / 4) + 7) / 8) + 64) * 8;   // p.Ap partials
     long maxb = std::max<long>(cells_launch(g0, g->dim).nblocks(), 2048);
@@ -564,10 +613,11 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     }
     // fused level-0 launch (at most) and the row kernels' workgroups
     {
-        const int TX = (g->dim == 3) ? 60 : 252, TY = (g->dim == 3) ? 12 : 1;
-        const long ta = (g0.nx + TX - 1) / TX, tb = (g->dim == 3) ? (g0.ny + TY - 1) / TY : 1;
-        const int nm = (g->dim == 3) ? g0.nz : g0.ny;
-        maxb = std::max(maxb, ta * tb * std::max(1, (nm + 7) / 8));
+        using E3 = Engine<3, float, float>;
+        using E2 = Engine<2, float, float>;
+        const dim3 mg = (g->dim == 3) ? E3::march_launch(g0, E3::RX0, E3::RY0, o->march_target, o->march_kc_min).grid
+                                      : E2::march_launch(g0, E2::RX0, E2::RY0, o->march_target, o->march_kc_min).grid;
+        maxb = std::max(maxb, (long)mg.x * mg.y * mg.z);
         if (g->dim == 3) {
             const long tiles = (long)((g0.ny + 15) / 16) * ((g0.nz + 3) / 4);
             maxb = std::max(maxb, 8 * ((tiles + 7) / 8));
@@ -633,7 +683,8 @@ extern "C" int uaamg_batch_create(uaamg_batch** out, const uaamg_opts* o) {
     std::string why;
     if (!opts_valid(o, why)) return UAAMG_ERR_ARG;
     if (o->precision != UAAMG_FP32 && o->precision != UAAMG_MIXED_BF16) return UAAMG_ERR_ARG;
-    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused) return UAAMG_ERR_ARG;
+    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused || !(o->forms & UAAMG_FORM_TAIL) ||
+        o->preconditioner != UAAMG_PRECOND_UAAMG) return UAAMG_ERR_ARG;
     int rc = check_device(why);
     if (rc) { std::fprintf(stderr, "uaamg_batch_create: %s\n", why.c_str()); return rc; }
     uaamg_batch* b = new (std::nothrow) uaamg_batch();
@@ -817,6 +868,7 @@ struct uaamg_server {
     std::vector<Request> reqs;
     std::deque<int> queue;
     std::vector<int> active;
+    std::deque<int> undelivered;       // finished requests not yet returned by a step (done[] was full)
     long tick = 0;
     uint8_t* stage = nullptr; long stage_n = 0;
 };
@@ -844,7 +896,8 @@ extern "C" int uaamg_server_create(uaamg_server** out, const uaamg_opts* o, size
     std::string why;
     if (!opts_valid(o, why)) return UAAMG_ERR_ARG;
     if (o->precision != UAAMG_FP32 && o->precision != UAAMG_MIXED_BF16) return UAAMG_ERR_ARG;
-    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused) return UAAMG_ERR_ARG;
+    if (o->nu_pre != 1 || o->nu_post != 1 || !o->fused || !(o->forms & UAAMG_FORM_TAIL) ||
+        o->preconditioner != UAAMG_PRECOND_UAAMG) return UAAMG_ERR_ARG;
     int rc = check_device(why);
     if (rc) { std::fprintf(stderr, "uaamg_server_create: %s\n", why.c_str()); return rc; }
     uaamg_server* s = new (std::nothrow) uaamg_server();
@@ -977,6 +1030,10 @@ extern "C" int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* 
             s->part[st.part]->start_member(st.slot, r.rhs, r.p, r.warm, r.max_iter, r.tol);
             s->active.push_back(id);
         }
+        // finished requests a previous step could not return (its done[] was full) come first
+        int nd = 0;
+        while (!s->undelivered.empty() && nd < max_done && done) { done[nd++] = s->undelivered.front(); s->undelivered.pop_front(); }
+        if (n_done) *n_done = nd;
         if (s->active.empty()) return UAAMG_OK;
         // one chunk of iterations for every active request, per batch part
         bool run[2] = {false, false};
@@ -984,8 +1041,7 @@ extern "C" int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* 
         for (int pi = 0; pi < 2; pi++) if (run[pi]) s->part[pi]->run_chunk();
         HIPCHK(hipStreamSynchronize(s->stream));
         // completions
-        std::vector<int> still;
-        int nd = 0;
+        std::vector<int> still, fin;
         for (int id : s->active) {
             Request& r = s->reqs[id];
             SimState& st = s->sims[r.sim];
@@ -1001,18 +1057,19 @@ extern "C" int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* 
             r.state = 2;
             st.busy = false;
             st.last_used = ++s->tick;
-            if (done && nd < max_done) done[nd] = id;
-            nd++;
+            fin.push_back(id);
         }
         s->active.swap(still);
-        if (nd) HIPCHK(hipStreamSynchronize(s->stream));
+        if (!fin.empty()) HIPCHK(hipStreamSynchronize(s->stream));
         const auto now = std::chrono::steady_clock::now();
-        for (int k = 0; k < nd && done && k < max_done; k++) {
-            Request& r = s->reqs[done[k]];
+        for (int id : fin) {   // every finished request gets its latency, returned now or by a later step
+            Request& r = s->reqs[id];
             r.t_done = now;
             r.st.ms_total = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+            if (done && nd < max_done) done[nd++] = id;
+            else s->undelivered.push_back(id);
         }
-        if (n_done) *n_done = std::min(nd, max_done);
+        if (n_done) *n_done = nd;
     })
     return UAAMG_OK;
 }

@@ -201,6 +201,48 @@ def poisson2d_dirichlet(seed=0, n=256, noise=1e-3):
     return pb
 
 
+def _smoothstep(t):
+    s = (t + 1.0) * 0.5
+    return np.where(t <= -1.0, 0.0, np.where(t >= 1.0, 1.0, s * s * (3.0 - 2.0 * s)))
+
+
+def smooth_cylinder_lap(n, L, centre, R, tw=0.0):
+    """Variable Laplacian coefficients of a smooth cylinder, as
+    SmoothCylinder::applyToLaplacian writes them into Grid::lap_diag,
+    lap_off_x, lap_off_y (src/geometry/cylinder_model.cpp:50-71, the
+    solid fraction of :40-43 and smoothstep of :5-10): cells with solid
+    fraction > 0.9 become solid with diagonal 1e15, the others get face
+    permeabilities 1 - (phi + phi_nb)/2 + 0.1.  Returns (solid, diag, off_x,
+    off_y), padded 2D arrays (i fastest)."""
+    nx, ny = int(n[0]), int(n[1])
+    dx, dy = L[0] / nx, L[1] / ny
+    tw = tw if tw > 0 else dx
+    cx, cy = float(centre[0]), float(centre[1])
+    eps, large = 0.1, 1e15
+
+    def frac(x, y):
+        return _smoothstep((np.sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy)) - R) / tw)
+
+    j, i = np.meshgrid(np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    xc, yc = (i - 0.5) * dx, (j - 0.5) * dy
+    phi = frac(xc, yc)
+    pL, pR, pB, pT = frac(xc - dx, yc), frac(xc + dx, yc), frac(xc, yc - dy), frac(xc, yc + dy)
+    cL = 1.0 - 0.5 * (phi + pL) + eps
+    cR = 1.0 - 0.5 * (phi + pR) + eps
+    cB = 1.0 - 0.5 * (phi + pB) + eps
+    cT = 1.0 - 0.5 * (phi + pT) + eps
+    ix, iy = 1.0 / (dx * dx), 1.0 / (dy * dy)
+    sol = phi > 0.9
+    shape = (ny + 2, nx + 2)
+    out = [np.zeros(shape, np.uint8), np.zeros(shape), np.zeros(shape), np.zeros(shape)]
+    inner = (slice(1, -1), slice(1, -1))
+    out[0][inner] = sol
+    out[1][inner] = np.where(sol, large, (cL + cR) * ix + (cB + cT) * iy)
+    out[2][inner] = np.where(sol, 0.0, -0.5 * (cL + cR) * ix)
+    out[3][inner] = np.where(sol, 0.0, -0.5 * (cB + cT) * iy)
+    return tuple(a.reshape(-1) for a in out)
+
+
 def _random_mask(rng, n, frac_max=0.3):
     """Random boxes/spheres covering up to frac_max of the cells (padded, [k][j][i])."""
     dim = len(n)
@@ -309,3 +351,69 @@ def _consistent_rhs(pb):
     mean[touches] = 0.0
     rhs -= mean[L]
     rhs[~fl] = 0.0
+
+
+# ---- served stream: many concurrent simulations, one pressure solve per frame ----
+
+def serve_sims(nsim, seed, dev):
+    """Simulations of the served stream: half 2D (side log-uniform 64..2048),
+    half 3D (side log-uniform 32..256), all walls Neumann, up to 3 disjoint
+    solid balls away from the walls (the fluid stays connected), generated on
+    the device."""
+    import torch
+    rng = np.random.default_rng(seed)
+    sims = []
+    for k in range(nsim):
+        dim = 2 if k % 2 == 0 else 3
+        lo, hi = (64, 2048) if dim == 2 else (32, 256)
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        n = (side,) * dim
+        shape = tuple(v + 2 for v in n)[::-1]
+        ax = [((torch.arange(s, device=dev, dtype=torch.float32) - 0.5) / (s - 2)) for s in shape]
+        X = torch.meshgrid(*ax, indexing="ij")
+        solid = torch.zeros(shape, dtype=torch.bool, device=dev)
+        balls = []
+        for _ in range(int(rng.integers(0, 4))):
+            for _try in range(20):
+                r = rng.uniform(0.04, 0.1)
+                c = rng.uniform(r + 0.05, 1 - r - 0.05, dim)
+                if all(np.linalg.norm(c - c2) > r + r2 + 0.05 for c2, r2 in balls):
+                    balls.append((c, r))
+                    break
+        for c, r in balls:
+            d2 = sum((X[q] - float(c[dim - 1 - q])) ** 2 for q in range(dim))
+            solid |= d2 < r * r
+        inner = (slice(1, -1),) * dim
+        fluid = torch.zeros(shape, dtype=torch.bool, device=dev)
+        fluid[inner] = True
+        fluid &= ~solid
+        # smooth rhs: a few Gaussian sources and sinks, zero mean over the fluid
+        rhs = torch.zeros(shape, dtype=torch.float64, device=dev)
+        for _ in range(6):
+            c = rng.uniform(0.1, 0.9, dim)
+            s = rng.uniform(0.05, 0.2)
+            rhs += rng.uniform(-1e3, 1e3) * torch.exp(-sum((X[q].double() - float(c[dim - 1 - q])) ** 2 for q in range(dim)) / s ** 2)
+        rhs[~fluid] = 0
+        rhs[fluid] -= rhs[fluid].mean()
+        sims.append({"n": n, "h": tuple(1.0 / v for v in n), "solid": solid.reshape(-1).to(torch.uint8).cpu().numpy(),
+                     "fluid": fluid.reshape(-1), "rhs": rhs.reshape(-1), "shape": shape, "dim": dim})
+    return sims
+
+
+def next_frame(sim, rng):
+    """Frame t+1: frame t's rhs advected by a smooth (one-cell) shift plus 5 % new noise."""
+    import torch
+    r = sim["rhs"].reshape(sim["shape"])
+    ax = int(rng.integers(0, sim["dim"]))
+    r = torch.roll(r, shifts=int(rng.choice([-1, 1])), dims=ax)
+    gen = torch.Generator(device=r.device)
+    gen.manual_seed(int(rng.integers(0, 2 ** 62)))
+    noise = torch.randn(r.shape, dtype=torch.float64, device=r.device, generator=gen)
+    noise = noise * (0.05 * float(r.abs().max()))
+    r = r + noise
+    r = r.reshape(-1)
+    f = sim["fluid"]
+    r[~f] = 0
+    r[f] -= r[f].mean()
+    sim["rhs"] = r
+    return r
