@@ -207,6 +207,22 @@ int uaamg_level_data(uaamg_ctx* ctx, int level, uint8_t* solid, double* diag,
 int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, double tol,
                  int* iters, int precision);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc: tight serving loops, harnesses
+ * that forbid allocation): uaamg_amgpcg on device arrays, with every
+ * intermediate buffer in one fixed 32 MiB device array that the library holds
+ * on each GPU it runs on, from its first call there until the process ends.
+ * b_dev, x_dev: FP64 device arrays of (N+2)^2 values in uaamg_amgpcg's layout
+ * (x_dev is written whole); stream: a hipStream_t, or NULL for the default
+ * stream; the call returns when x_dev is written.  Sizes whose buffers exceed
+ * the array (N > 737 in FP64, N > 1043 in FP32) return UAAMG_ERR_ADMISSION.
+ * Allocates nothing: no device or pinned host memory, no stream or event.
+ * Every call on one GPU uses that GPU's one array and clears it first: calls
+ * that overlap in time on one GPU (from other host threads or streams) race,
+ * so run them one after another. */
+int uaamg_amgpcg_noalloc(int N, const double* b_dev, double* x_dev, int max_iter, double tol,
+                         int* iters, int precision, void* stream);
+
 /* Pressure projection around the solve: replaces PressureProjection::project
  * (src/pressure/pressure.cpp:5-32) and PressureProjection3D::project
  * (src/pressure/pressure_3d.cpp:5-39) with the pcg_uaamg solver.  On the grid

@@ -169,3 +169,18 @@ def test_host_mirror_keys():
     r = subprocess.run([HOST_TOOL, "keys"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert "keys ok" in r.stdout
+
+
+def test_amgpcg_noalloc_rejects_oversize_before_the_device():
+    """uaamg_amgpcg_noalloc refuses sizes whose buffers exceed its fixed
+    device array (N > 737 in FP64, N > 1043 in FP32) with
+    UAAMG_ERR_ADMISSION before touching any device; bad arguments first."""
+    L = U.load().L
+    it = C.c_int(-1)
+    dummy = C.c_void_p(16)        # never dereferenced: the call is refused first
+    assert L.uaamg_amgpcg_noalloc(738, dummy, dummy, 10, 1e-6, C.byref(it), U.FP64, None) == U.ERR_ADMISSION
+    assert L.uaamg_amgpcg_noalloc(1044, dummy, dummy, 10, 1e-6, C.byref(it), U.FP32, None) == U.ERR_ADMISSION
+    assert L.uaamg_amgpcg_noalloc(1, dummy, dummy, 10, 1e-6, C.byref(it), U.FP64, None) == U.ERR_ARG
+    assert L.uaamg_amgpcg_noalloc(64, None, dummy, 10, 1e-6, C.byref(it), U.FP64, None) == U.ERR_ARG
+    assert L.uaamg_amgpcg_noalloc(64, dummy, dummy, 10, 1e-6, C.byref(it), U.MIXED_BF16, None) == U.ERR_ARG
+    assert it.value == -1

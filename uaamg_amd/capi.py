@@ -49,7 +49,7 @@ class Stats(C.Structure):
 EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_error", "uaamg_set_opts",
            "uaamg_set_grid", "uaamg_set_grid_device", "uaamg_set_variable_lap", "uaamg_solve", "uaamg_solve_device",
            "uaamg_apply_precond", "uaamg_matvec", "uaamg_hierarchy", "uaamg_level_data",
-           "uaamg_amgpcg", "uaamg_bytes_required", "uaamg_project", "uaamg_project_device",
+           "uaamg_amgpcg", "uaamg_amgpcg_noalloc", "uaamg_bytes_required", "uaamg_project", "uaamg_project_device",
            "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
            "uaamg_batch_add_device", "uaamg_batch_clear", "uaamg_batch_size", "uaamg_batch_bytes",
            "uaamg_batch_solve",
@@ -86,6 +86,7 @@ class Lib:
         L.uaamg_hierarchy.argtypes = [P, P, P, I]
         L.uaamg_level_data.argtypes = [P, I, P, P, P, P, P, P]
         L.uaamg_amgpcg.argtypes = [I, P, P, I, D, P, I]
+        L.uaamg_amgpcg_noalloc.argtypes = [I, P, P, I, D, P, I, P]
         L.uaamg_bytes_required.argtypes = [P, P]
         L.uaamg_bytes_required.restype = C.c_size_t
         L.uaamg_project.argtypes = [P, P, P, P, P, D, I, D, P]

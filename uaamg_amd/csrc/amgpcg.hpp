@@ -108,46 +108,64 @@ __global__ void k_ag_pupd(T* __restrict__ p, const T* __restrict__ z, int N, dou
     p[id] = (T)((double)z[id] + beta * (double)p[id]);
 }
 
-template <typename T>
-int amgpcg_run(int N, const double* b_host, double* x_host, int max_iter, double tol, int* iters) {
+// Level sizes of the amgpcg hierarchy: N, N/2, ... down to <= 3 (test_amgpcg.cpp:66-71)
+inline std::vector<int> amgpcg_levels(int N) {
     std::vector<int> lv;
     for (int n = N; n >= 2;) { lv.push_back(n); if (n <= 3) break; n = n / 2; if (n < 2) n = 2; }
+    return lv;
+}
+
+// Device buffers of one amgpcg solve, carved in order from one arena (each
+// start 256-byte aligned): per level x and b, then x, r, z, p, Ap, the dot
+// partials and the scalar slot.
+constexpr size_t kAgAlign = 256;
+inline size_t ag_round(size_t b) { return (b + kAgAlign - 1) / kAgAlign * kAgAlign; }
+inline dim3 ag_grid(int n) { return dim3((n + 63) / 64, (n + 3) / 4); }
+
+template <typename T>
+size_t amgpcg_arena_bytes(int N) {
+    size_t bytes = 0;
+    for (int n : amgpcg_levels(N)) bytes += 2 * ag_round((size_t)(n + 2) * (n + 2) * sizeof(T));
+    const size_t NN = (size_t)(N + 2) * (N + 2);
+    bytes += 5 * ag_round(NN * sizeof(T));
+    const long maxb = (long)ag_grid(N).x * ag_grid(N).y;
+    bytes += ag_round((maxb + 64) * sizeof(double)) + ag_round(64);
+    return bytes;
+}
+
+// The solve on device arrays: b_dev (FP64, (N+2)^2 row-major, interior read)
+// and x_dev (FP64, (N+2)^2, written whole).  `arena` holds at least
+// amgpcg_arena_bytes<T>(N) bytes; it is cleared here.  The host reads each dot
+// product (hscal: a host double, pinned or not) as the reference loop does.
+template <typename T>
+int amgpcg_core(int N, const double* b_dev, double* x_dev, int max_iter, double tol, int* iters,
+                unsigned char* arena, size_t arena_bytes, hipStream_t s, double* hscal) {
+    const std::vector<int> lv = amgpcg_levels(N);
     const int nl = (int)lv.size();
-    hipStream_t s;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    std::vector<void*> allocs;
-    auto dal = [&](size_t bytes) {
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) throw HipError{e, "hipMalloc"};
-        allocs.push_back(p);
-        HIPCHK(hipMemsetAsync(p, 0, bytes, s));
+    const size_t need = amgpcg_arena_bytes<T>(N);
+    if (need > arena_bytes) throw HipError{hipErrorInvalidValue, "amgpcg arena too small"};
+    HIPCHK(hipMemsetAsync(arena, 0, need, s));
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        void* p = arena + used;
+        used += ag_round(bytes);
         return p;
     };
-    struct Cleanup {
-        std::vector<void*>& a; hipStream_t s;
-        ~Cleanup() { (void)hipStreamSynchronize(s); for (void* p : a) (void)hipFree(p); (void)hipStreamDestroy(s); }
-    } cleanup{allocs, s};
     const size_t NN = (size_t)(N + 2) * (N + 2);
     std::vector<T*> X(nl), B(nl);
     for (int l = 0; l < nl; l++) {
         size_t n2 = (size_t)(lv[l] + 2) * (lv[l] + 2);
-        X[l] = (T*)dal(n2 * sizeof(T)); B[l] = (T*)dal(n2 * sizeof(T));
+        X[l] = (T*)take(n2 * sizeof(T)); B[l] = (T*)take(n2 * sizeof(T));
     }
-    T *x = (T*)dal(NN * sizeof(T)), *r = (T*)dal(NN * sizeof(T)), *z = (T*)dal(NN * sizeof(T));
-    T *p = (T*)dal(NN * sizeof(T)), *Ap = (T*)dal(NN * sizeof(T));
-    double* bd = (double*)dal(NN * sizeof(double));
+    T *x = (T*)take(NN * sizeof(T)), *r = (T*)take(NN * sizeof(T)), *z = (T*)take(NN * sizeof(T));
+    T *p = (T*)take(NN * sizeof(T)), *Ap = (T*)take(NN * sizeof(T));
     dim3 blk(64, 4);
-    auto grd = [](int n) { return dim3((n + 63) / 64, (n + 3) / 4); };
+    auto grd = ag_grid;
     long maxb = (long)grd(N).x * grd(N).y;
-    double* partials = (double*)dal((maxb + 64) * sizeof(double));
-    double* scal = (double*)dal(64);
-    double* hscal = nullptr;
-    HIPCHK(hipHostMalloc((void**)&hscal, 64, hipHostMallocDefault));
-    struct HostFree { double* p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{hscal};
+    double* partials = (double*)take((maxb + 64) * sizeof(double));
+    double* scal = (double*)take(64);
 
-    HIPCHK(hipMemcpyAsync(bd, b_host, NN * sizeof(double), hipMemcpyHostToDevice, s));
-    k_cast_all_ag<T><<<(unsigned)((NN + 255) / 256), 256, 0, s>>>(bd, r, (long)NN, N);
+    k_cast_all_ag<T><<<(unsigned)((NN + 255) / 256), 256, 0, s>>>(b_dev, r, (long)NN, N);
 
     auto h_of = [](int n) { return 1.0 / (n + 1); };
     auto rbgs = [&](int l, int sweeps) {
@@ -205,10 +223,43 @@ int amgpcg_run(int N, const double* b_host, double* x_host, int max_iter, double
         rsold = rsnew;
     }
     HIPCHK(hipGetLastError());
-    k_cast_all_ag_out<T><<<(unsigned)((NN + 255) / 256), 256, 0, s>>>(x, bd, (long)NN);
-    HIPCHK(hipMemcpyAsync(x_host, bd, NN * sizeof(double), hipMemcpyDeviceToHost, s));
+    k_cast_all_ag_out<T><<<(unsigned)((NN + 255) / 256), 256, 0, s>>>(x, x_dev, (long)NN);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     *iters = it_out;
+    return UAAMG_OK;
+}
+
+// Host arrays in and out: the arena and a b/x staging array are allocated
+// per call on a stream of its own.
+template <typename T>
+int amgpcg_run(int N, const double* b_host, double* x_host, int max_iter, double tol, int* iters) {
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<void*> allocs;
+    auto dal = [&](size_t bytes) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+        allocs.push_back(p);
+        return p;
+    };
+    struct Cleanup {
+        std::vector<void*>& a; hipStream_t s;
+        ~Cleanup() { (void)hipStreamSynchronize(s); for (void* p : a) (void)hipFree(p); (void)hipStreamDestroy(s); }
+    } cleanup{allocs, s};
+    const size_t NN = (size_t)(N + 2) * (N + 2);
+    const size_t ab = amgpcg_arena_bytes<T>(N);
+    unsigned char* arena = (unsigned char*)dal(ab);
+    double* bd = (double*)dal(NN * sizeof(double));
+    double* hscal = nullptr;
+    HIPCHK(hipHostMalloc((void**)&hscal, 64, hipHostMallocDefault));
+    struct HostFree { double* p; ~HostFree() { if (p) (void)hipHostFree(p); } } hf{hscal};
+    HIPCHK(hipMemcpyAsync(bd, b_host, NN * sizeof(double), hipMemcpyHostToDevice, s));
+    // bd holds b until the core has cast it into r, then receives x
+    amgpcg_core<T>(N, bd, bd, max_iter, tol, iters, arena, ab, s, hscal);
+    HIPCHK(hipMemcpyAsync(x_host, bd, NN * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return UAAMG_OK;
 }
 

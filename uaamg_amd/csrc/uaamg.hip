@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include "../../include/uaamg.h"
 #include "kernels_proj.hpp"
@@ -639,6 +640,70 @@ extern "C" int uaamg_amgpcg(int N, const double* b, double* x, int max_iter, dou
     } catch (const HipError& e) {
         std::fprintf(stderr, "uaamg_amgpcg: %s (%s)\n", hipGetErrorString(e.e), e.where.c_str());
         return e.e == hipErrorOutOfMemory ? UAAMG_ERR_OOM : UAAMG_ERR_HIP;
+    }
+}
+
+// ------------------------------------------------------------------ amgpcg without allocation
+//
+// Every intermediate buffer of uaamg_amgpcg_noalloc lives in `scratch`, a
+// fixed device array each GPU holds from the library's load until the process
+// ends.  In host code `scratch` names a host-side stand-in, not the device
+// memory: the device address comes from hipGetSymbolAddress, once per device
+// (with that device current), and only that address is passed to kernels and
+// copies.  Calls on one GPU share its array: calls that overlap in time on one
+// GPU race.
+namespace {
+constexpr size_t kAmgScratchBytes = 32u << 20;
+__device__ __align__(256) unsigned char scratch[kAmgScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on its first use there)
+int scratch_on_device(unsigned char** out, std::string& why) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) { why = "hipGetDevice failed"; return UAAMG_ERR_HIP; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return UAAMG_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return UAAMG_OK;
+}
+}  // namespace
+
+extern "C" int uaamg_amgpcg_noalloc(int N, const double* b_dev, double* x_dev, int max_iter, double tol,
+                                    int* iters, int precision, void* stream) {
+    if (N < 2 || !b_dev || !x_dev || !iters) return UAAMG_ERR_ARG;
+    if (precision != UAAMG_FP64 && precision != UAAMG_FP32) return UAAMG_ERR_ARG;
+    const size_t need = precision == UAAMG_FP64 ? amgpcg_arena_bytes<double>(N) : amgpcg_arena_bytes<float>(N);
+    if (need > kAmgScratchBytes) {
+        std::fprintf(stderr, "uaamg_amgpcg_noalloc: N = %d needs %zu bytes of scratch, the fixed device array holds %zu "
+                     "(N <= 737 in FP64, N <= 1043 in FP32); use uaamg_amgpcg\n", N, need, kAmgScratchBytes);
+        return UAAMG_ERR_ADMISSION;
+    }
+    std::string why;
+    int rc = check_device(why);
+    if (rc) { std::fprintf(stderr, "uaamg_amgpcg_noalloc: %s\n", why.c_str()); return rc; }
+    unsigned char* ptr = nullptr;
+    rc = scratch_on_device(&ptr, why);
+    if (rc) { std::fprintf(stderr, "uaamg_amgpcg_noalloc: %s\n", why.c_str()); return rc; }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double hscal[8] = {0.0};   // dot products land in host stack memory (no pinned allocation)
+    try {
+        return precision == UAAMG_FP64
+            ? amgpcg_core<double>(N, b_dev, x_dev, max_iter, tol, iters, ptr, kAmgScratchBytes, s, hscal)
+            : amgpcg_core<float>(N, b_dev, x_dev, max_iter, tol, iters, ptr, kAmgScratchBytes, s, hscal);
+    } catch (const HipError& e) {
+        std::fprintf(stderr, "uaamg_amgpcg_noalloc: %s (%s)\n", hipGetErrorString(e.e), e.where.c_str());
+        return UAAMG_ERR_HIP;
     }
 }
 
