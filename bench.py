@@ -57,9 +57,11 @@ def per_cell_bytes(cls, sv, sl, dim, mode):
 def cpu_baseline(pb, k_iters, budget_s=30.0):
     """Time the reference CPU solver (oracle/_ref, compiled from the reference
     sources) or, when absent, the oracle restatement, single thread, on a
-    bounded sample of the same workload: solves with max_iter = 1 and 2 give
-    t_iter = t2 - t1 and t_setup = t1 - t_iter; a full solve is extrapolated as
-    t_setup + k * t_iter with k the GPU solve's iteration count."""
+    bounded sample of the same workload: solves with max_iter = 1 and 3 give
+    t_iter = (t3 - t1) / 2 and t_setup = t1 - t_iter; a full solve is
+    extrapolated as t_setup + k * t_iter with k the reference's own iteration
+    count for relres 1e-6 (the FP64 GPU solve's, which equals the oracle's
+    within one)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     kind = "reference" if ob.ref_available() else "port"
@@ -75,8 +77,8 @@ def cpu_baseline(pb, k_iters, budget_s=30.0):
         return time.perf_counter() - t
 
     t1 = run(1)
-    t2 = run(2)
-    t_iter = max(t2 - t1, 1e-9)
+    t3 = run(3)
+    t_iter = max((t3 - t1) / 2, 1e-9)
     t_setup = max(t1 - t_iter, 0.0)
     t_solve = t_setup + k_iters * t_iter
     cpu = platform.processor() or platform.machine()
@@ -94,11 +96,92 @@ def cpu_baseline(pb, k_iters, budget_s=30.0):
         "cores": 1,
         "kind": kind,
         "sample": (f"{'oracle/_ref (reference PCG3D+UAAMGPreconditioner3D, g++ -O2)' if kind == 'reference' else 'oracle restatement (C, gcc -O2)'}"
-                   f" FP64 1 thread on {pb.name}: max_iter=1 took {t1:.2f}s, max_iter=2 took {t2:.2f}s;"
-                   f" solve extrapolated as setup {t_setup:.2f}s + {k_iters} x {t_iter:.2f}s/iter;"
+                   f" FP64 1 thread on {pb.name}: max_iter=1 took {t1:.2f}s, max_iter=3 took {t3:.2f}s;"
+                   f" solve extrapolated as setup {t_setup:.2f}s + {k_iters} x {t_iter:.2f}s/iter"
+                   f" ({k_iters} = the FP64 iteration count at relres 1e-6);"
                    f" host {cpu}, nproc {os.cpu_count()}"),
         "s_per_iter": t_iter,
     }
+
+
+def pmc_summary(fetch_csv, write_csv, out=sys.stdout):
+    """Per-dispatch and per-step HBM bytes from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, counter_collection.csv) of
+    `bench.py --steps K --warmup W --copy-calibration ...`.  Both counters
+    are scaled by what they read for the calibration's 256 MiB device copies
+    (the largest __amd_rocclr_copyBuffer dispatches).  A step (set-up from
+    the mask, zeroing of p, the solve) begins at a k_nbr0 dispatch; the
+    per-step total sums every dispatch from the first k_nbr0 on, divided by
+    the number of solves (k_ctrl_init dispatches)."""
+    import csv
+
+    def load(path):
+        rows = list(csv.DictReader(open(path)))
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        return rows
+
+    fr, wr = load(fetch_csv), load(write_csv)
+    if len(fr) != len(wr):
+        raise SystemExit(f"dispatch counts differ: {len(fr)} vs {len(wr)}")
+    calib = 268435456.0
+    cf = [float(r["Counter_Value"]) for r in fr if r["Kernel_Name"] == "__amd_rocclr_copyBuffer"]
+    cw = [float(r["Counter_Value"]) for r in wr if r["Kernel_Name"] == "__amd_rocclr_copyBuffer"]
+    big_f, big_w = sorted(cf)[-3:], sorted(cw)[-3:]
+    kf = calib / (np.mean(big_f) * 1024.0)
+    kw = calib / (np.mean(big_w) * 1024.0)
+    per = {}
+    start = next(i for i, r in enumerate(fr) if "k_nbr0" in r["Kernel_Name"])
+    nsolve = sum(1 for r in fr[start:] if "k_ctrl_init" in r["Kernel_Name"])
+    tot_f = tot_w = 0.0
+    for a, b in zip(fr[start:], wr[start:]):
+        f = float(a["Counter_Value"]) * 1024.0 * kf
+        w = float(b["Counter_Value"]) * 1024.0 * kw
+        tot_f += f
+        tot_w += w
+        name = a["Kernel_Name"].split("(")[0]
+        key = (name, int(a["Grid_Size"]))
+        e = per.setdefault(key, [0, 0.0, 0.0])
+        e[0] += 1; e[1] += f; e[2] += w
+    print(f"calibration: the 3 copies of 268435456 bytes read FETCH_SIZE {np.mean(big_f):.1f} KiB and WRITE_SIZE "
+          f"{np.mean(big_w):.1f} KiB per dispatch, so bytes = FETCH_SIZE x {kf * 1024:.1f} and "
+          f"WRITE_SIZE x {kw * 1024:.1f} per KiB", file=out)
+    print(f"steps counted: {nsolve} (dispatches {len(fr) - start})", file=out)
+    print(f"per step: fetch {tot_f / nsolve / 1e6:.1f} MB, write {tot_w / nsolve / 1e6:.1f} MB, "
+          f"total {(tot_f + tot_w) / nsolve / 1e6:.1f} MB", file=out)
+    print("", file=out)
+    print("| kernel | threads | dispatches per step | fetch MB / dispatch | write MB / dispatch | total MB per step |", file=out)
+    print("|---|---:|---:|---:|---:|---:|", file=out)
+    for (name, grid), (n, f, w) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
+        if (f + w) / nsolve < 1e6:
+            continue
+        print(f"| `{name[:90]}` | {grid} | {n / nsolve:.1f} | {f / n / 1e6:.1f} | {w / n / 1e6:.1f} | "
+              f"{(f + w) / nsolve / 1e6:.1f} |", file=out)
+    return (tot_f + tot_w) / nsolve
+
+
+def trace_summary(stats_csv, trace_csv, top=20, out=sys.stdout):
+    """Markdown tables of a rocprofv3 --kernel-trace --stats run: the top
+    kernels by total time, and the average duration per kernel and grid size
+    (one row per level for the kernels every level runs)."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    print("| kernel | calls | avg us | total ms | % |", file=out)
+    print("|---|---:|---:|---:|---:|", file=out)
+    for r in rows[:top]:
+        print(f"| `{r['Name'].split('(')[0][:90]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+              f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |", file=out)
+    print(f"\nSum of kernel time: {sum(float(r['TotalDurationNs']) for r in rows) / 1e6:.1f} ms.\n", file=out)
+    per = {}
+    for r in csv.DictReader(open(trace_csv)):
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        key = (r["Kernel_Name"].split("(")[0], grid, r["VGPR_Count"], r["LDS_Block_Size"])
+        e = per.setdefault(key, [0, 0.0])
+        e[0] += 1
+        e[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+    print("| kernel | threads | VGPRs | LDS bytes | launches | avg us |", file=out)
+    print("|---|---:|---:|---:|---:|---:|", file=out)
+    for (name, grid, vg, lds), (n, t) in sorted(per.items(), key=lambda kv: -kv[1][1])[:top]:
+        print(f"| `{name[:90]}` | {grid} | {vg} | {lds} | {n} | {t / n:.2f} |", file=out)
 
 
 def load_traffic(workload):
@@ -184,7 +267,21 @@ def bench_batch(args):
     log(f"[batch] batched fp32: {t_b32 * 1e3:.1f} ms")
     t_bbf, out_bbf, it_bbf, rcbf = batched(U.MIXED_BF16, reps)
     log(f"[batch] batched bf16 levels: {t_bbf * 1e3:.1f} ms")
-    _, out_64, _ = plain(U.FP64, 0, 1)
+    # converged FP64 references (relres 1e-10); the rhs is scaled by a power of
+    # two (exact) so the absolute p.Ap < 1e-15 guard of the reference loop
+    # cannot end a solve early
+    out_64 = []
+    for k, p in enumerate(pbs):
+        sc = 2.0 ** float(np.round(np.log2(1e8 / max(float(rhs[k].norm()), 1e-300))))
+        c = U.Context(precision=U.FP64, tol_mode=U.TOL_REL)
+        c.set_grid(p.n, p.h, p.solid, p.wall)
+        q = torch.zeros(p.N, dtype=torch.float64, device=dev)
+        rs = rhs[k] * sc
+        torch.cuda.synchronize()
+        c.solve_device(rs.data_ptr(), q.data_ptr(), 0, 5000, 1e-10)
+        torch.cuda.synchronize()
+        c.close()
+        out_64.append(q / sc)
 
     def err(outs):
         e = 0.0
@@ -210,7 +307,7 @@ def bench_batch(args):
         "batched_bf16_solves_per_s": round(n / t_bbf, 2),
         "speedup_bf16_batched_vs_best_plain": round(best_plain / t_bbf, 3),
         "max_rel_err_vs_fp64": {"plain_fp32": err(out_plain), "batched_fp32": err(out_b32),
-                                "batched_bf16": err(out_bbf)},
+                                "batched_bf16": err(out_bbf), "reference": "FP64 at relres 1e-10"},
         "iters_max": {"plain_fp32": int(max(it_plain)), "batched_fp32": int(max(it_b32)),
                       "batched_bf16": int(max(it_bbf))},
         "status": {"batched_fp32": U.status_name(rc32), "batched_bf16": U.status_name(rcbf)},
@@ -436,11 +533,21 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--no-project", action="store_true", help="skip the projection measurement")
     ap.add_argument("--project-only", action="store_true", help="only the projection measurement")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this benchmark and exit")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "KERNEL_TRACE_CSV"),
+                    help="tabulate a rocprofv3 --kernel-trace --stats run of this benchmark and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (3x), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
 
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py: the benchmark runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent solves shard per GPU with no communication; a single solve would need z-slab halos.", file=sys.stderr)
@@ -539,6 +646,7 @@ def main():
     bytes_launch = per_cell_bytes(dname, sv, sl, pb.dim, mode) * pb.cells
     achieved = bytes_launch / avg_s / 1e9
     traffic = load_traffic(f"{pb.name}:{args.precision}:{mode}:{dname}")
+    step_bytes = load_traffic(f"{pb.name}:{args.precision}:step")   # rocprof FETCH+WRITE of one whole step
 
     k_med = int(np.median(iters))
     out = {
@@ -580,6 +688,9 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": bytes_launch,
             "avg_launch_us": round(avg_s * 1e6, 2),
+            # whole step (set-up, zeroing, solve) as rocprof counts it, over this run's step time
+            "measured_bytes_per_step": step_bytes,
+            "measured_gbs_solve": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1) if step_bytes else None,
         },
         "cpu_baseline": None,
     }
@@ -594,7 +705,12 @@ def main():
         out["serving"] = bench_serve(args)
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(pb, k_med)
+            # the reference's own iteration count: the FP64 form's (= the oracle's within one)
+            c64 = U.Context(precision=U.FP64, tol_mode=U.TOL_REL)
+            c64.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+            _, st64 = c64.solve(pb.rhs, 1000, 1e-6)
+            c64.close()
+            out["cpu_baseline"] = cpu_baseline(pb, st64.iters)
         except Exception as e:   # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
     print(json.dumps(out), flush=True)
