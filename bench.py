@@ -317,6 +317,46 @@ def bench_batch(args):
     return res
 
 
+def bench_keys(args):
+    """The 1024 x 512 MAC pressure solve (solid disc, relres 1e-6) under each
+    PCG key of the 2D factory (factory.cpp:19-23): pcg_uaamg, pcg_gmg (= pcg),
+    pcg_amg and cg, FP64 and FP32.  Per key and precision: iterations and the
+    median ms per solve over args.steps solves (inputs resident, grid set
+    once, p zeroed before each solve)."""
+    import torch
+    import uaamg_amd as U
+    from uaamg_amd import workloads as W
+    pb = W.pressure2d_mac(seed=0)
+    dev = torch.device("cuda:0")
+    rhs_d = torch.from_numpy(pb.rhs).to(dev)
+    p_d = torch.zeros(pb.N, dtype=torch.float64, device=dev)
+    keys = [("pcg_uaamg", U.PRECOND_UAAMG), ("pcg_gmg", U.PRECOND_GMG), ("pcg_amg", U.PRECOND_AMG), ("cg", U.PRECOND_NONE)]
+    rows = []
+    for key, pc in keys:
+        for pname, prec in (("fp64", U.FP64), ("fp32", U.FP32)):
+            mi = 20000 if pc == U.PRECOND_NONE else 2000
+            ctx = U.Context(precision=prec, tol_mode=U.TOL_REL, preconditioner=pc, use_graph=1)
+            ctx.set_grid(pb.n, pb.h, pb.solid, pb.wall)
+            times, st = [], None
+            for k in range(args.warmup + args.steps):
+                p_d.zero_()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                st = ctx.solve_device(rhs_d.data_ptr(), p_d.data_ptr(), 0, mi, 1e-6)
+                torch.cuda.synchronize()
+                if k >= args.warmup:
+                    times.append(1e3 * (time.perf_counter() - t0))
+            ctx.close()
+            ms = float(np.median(times))
+            row = {"key": key, "precision": pname, "iters": st.iters, "status": U.status_name(st.status),
+                   "ms_per_solve": round(ms, 3), "us_per_iter": round(1e3 * ms / max(1, st.iters), 1)}
+            log(f"[bench] keys {key:9s} {pname}: {st.iters:5d} iterations {ms:9.2f} ms/solve "
+                f"{row['us_per_iter']:8.1f} us/iteration ({row['status']})")
+            rows.append(row)
+    return {"workload": f"{pb.name}: 2D MAC pressure solve 1024x512, solid disc, relres 1e-6, hipGraph chunks",
+            "steps": args.steps, "rows": rows}
+
+
 def bench_project(args):
     """Pressure projection around the solve (PressureProjection3D::project,
     pressure_3d.cpp:5-39) at 256^3: the plume's MAC velocities (the solve
@@ -531,6 +571,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--keys-only", action="store_true",
+                    help="only the 1024x512 2D solve under each PCG key (pcg_uaamg, pcg_gmg, pcg_amg, cg)")
     ap.add_argument("--no-project", action="store_true", help="skip the projection measurement")
     ap.add_argument("--project-only", action="store_true", help="only the projection measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -561,6 +603,9 @@ def main():
         return
     if args.project_only:
         print(json.dumps({"projection": bench_project(args)}), flush=True)
+        return
+    if args.keys_only:
+        print(json.dumps({"keys_2d": bench_keys(args)}), flush=True)
         return
 
     import torch

@@ -82,12 +82,18 @@ typedef struct {
     int march_kc_min;          /* fewest planes per march-kernel chunk, even (default 2) */
     int march_target;          /* workgroups a march-kernel launch aims at (default 1024) */
     int trace;                 /* 1: relative residual history of each solve to stderr */
-    int preconditioner;        /* UAAMG_PRECOND_UAAMG (key "pcg_uaamg", factory.cpp:23) or
+    int preconditioner;        /* UAAMG_PRECOND_UAAMG (key "pcg_uaamg", factory.cpp:23),
                                   UAAMG_PRECOND_NONE (key "cg": IdentityPreconditioner, z = r,
-                                  factory.cpp:19, factory_3d.cpp:19); changing it drops the grid */
+                                  factory.cpp:19, factory_3d.cpp:19), UAAMG_PRECOND_GMG (keys "pcg"
+                                  and "pcg_gmg": GMGPreconditioner, factory.cpp:20-21) or
+                                  UAAMG_PRECOND_AMG (key "pcg_amg": AMGPreconditioner,
+                                  factory.cpp:22).  GMG and AMG exist for 2D grids only (the 3D
+                                  factory has no such keys, factory_3d.cpp:21-23: a 3D grid is
+                                  UAAMG_ERR_ARG) and run V(2,2) whatever nu_pre / nu_post say
+                                  (gmg_preconditioner.cpp:163-174); changing it drops the grid */
 } uaamg_opts;
 
-enum { UAAMG_PRECOND_UAAMG = 0, UAAMG_PRECOND_NONE = 1 };
+enum { UAAMG_PRECOND_UAAMG = 0, UAAMG_PRECOND_NONE = 1, UAAMG_PRECOND_GMG = 2, UAAMG_PRECOND_AMG = 3 };
 
 /* kernel forms (uaamg_opts.forms, uaamg_stats.forms_used) */
 enum {
@@ -240,7 +246,8 @@ int uaamg_project_device(uaamg_ctx* ctx, double* u, double* v, double* w, double
 int uaamg_project(uaamg_ctx* ctx, double* u, double* v, double* w, double* p, double dt,
                   int max_iter, double tol, uaamg_stats* st);
 
-/* Device memory (bytes) a context would hold for grid g under opts o. */
+/* Device memory (bytes) a context would hold for grid g under opts o; 0 when
+ * uaamg_set_grid would reject the pair (invalid grid or options). */
 size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o);
 
 /* ---- batches: many independent solves in one set of launches ----

@@ -154,7 +154,9 @@ inline void check(uaamg_ctx* h, int rc, const char* what) {
 // PCG with the UAAMG preconditioner on the GPU, behind Solver.  opts default
 // to the reference (FP64, V(1,1), absolute tolerance); pass UAAMG_FP32 etc.
 // HipPCGSolver(false) is plain CG (the identity preconditioner), as
-// CudaPCGSolver(use_precond) (cuda_pcg_solver.h:16) and factory key "cg".
+// CudaPCGSolver(use_precond) (cuda_pcg_solver.h:16) and factory key "cg";
+// opts.preconditioner = UAAMG_PRECOND_GMG / _AMG gives keys "pcg" /
+// "pcg_gmg" / "pcg_amg" (factory.cpp:20-22).
 // A Grid with a variable Laplacian (has_variable_lap) is solved with it, as
 // PCG::solve does (pcg.cpp:36-46).
 class HipPCGSolver : public Solver {
@@ -183,7 +185,12 @@ public:
         last_ = st;
     }
     std::string name() const override {
-        return opts_.preconditioner == UAAMG_PRECOND_NONE ? "CG(HIP)" : "PCG(HIP-UAAMG)";
+        switch (opts_.preconditioner) {
+            case UAAMG_PRECOND_NONE: return "CG(HIP)";
+            case UAAMG_PRECOND_GMG: return "PCG(HIP-GMG)";
+            case UAAMG_PRECOND_AMG: return "PCG(HIP-AMG)";
+            default: return "PCG(HIP-UAAMG)";
+        }
     }
     const uaamg_stats& last_stats() const { return last_; }
     int last_status() const { return last_.status; }
@@ -258,19 +265,33 @@ private:
     uaamg_stats last_{};
 };
 
-// Factory::create of this library: "pcg_uaamg" (factory.cpp:23) and "cg"
-// (PCG with the identity preconditioner, factory.cpp:19, factory_3d.cpp:19)
-// on the HIP backend have back-ends here; every other key (or the CPU
-// backend) throws instead of the reference's silent fallback to Jacobi
-// (factory.cpp:24), so a misspelled key cannot quietly select a different
-// solver.  Factory3D maps "pcg_uaamg" to the UAAMG solve (the reference's
-// factory_3d.cpp:26-27 falls back to Jacobi3D for it).
+// Factory::create of this library: the PCG keys "pcg_uaamg", "cg", "pcg",
+// "pcg_gmg" and "pcg_amg" (factory.cpp:19-23) on the HIP backend have
+// back-ends here; every other key (or the CPU backend) throws instead of the
+// reference's silent fallback to Jacobi (factory.cpp:24), so a misspelled key
+// cannot quietly select a different solver.  Factory3D has "pcg_uaamg" and
+// "cg" (the reference's factory_3d.cpp:26-27 falls back to Jacobi3D for
+// "pcg_uaamg" and has no GMG / AMG).
+inline int preconditioner_of(SolverKind k) {
+    switch (k) {
+        case SolverKind::CG: return UAAMG_PRECOND_NONE;
+        case SolverKind::PCG: case SolverKind::PCG_GMG: return UAAMG_PRECOND_GMG;   // factory.cpp:20-21
+        case SolverKind::PCG_AMG: return UAAMG_PRECOND_AMG;
+        case SolverKind::PCG_UAAMG: return UAAMG_PRECOND_UAAMG;
+        default: return -1;
+    }
+}
 struct Factory {
     static std::unique_ptr<Solver> create(const std::string& key, Backend be = Backend::HIP,
                                           const uaamg_opts* o = nullptr) {
-        const SolverKind k = parse_solver_kind(key);
-        if (be == Backend::HIP && (k == SolverKind::PCG_UAAMG || k == SolverKind::CG))
-            return std::make_unique<HipPCGSolver>(k == SolverKind::PCG_UAAMG, o);
+        const int pc = preconditioner_of(parse_solver_kind(key));
+        if (be == Backend::HIP && pc >= 0) {
+            uaamg_opts oo;
+            uaamg_default_opts(&oo);
+            if (o) oo = *o;
+            oo.preconditioner = pc;
+            return std::make_unique<HipPCGSolver>(&oo);
+        }
         throw std::invalid_argument("solver key '" + key + "' has no back-end in this library");
     }
 };

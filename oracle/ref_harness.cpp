@@ -14,6 +14,8 @@
 #include "solver/preconditioner/uaamg_preconditioner_3d.h"
 #include "solver/preconditioner/identity_preconditioner.h"
 #include "solver/preconditioner/identity_preconditioner_3d.h"
+#include "solver/preconditioner/gmg_preconditioner.h"
+#include "solver/preconditioner/amg_preconditioner.h"
 #include "geometry/cylinder_model.h"
 #include "pressure/pressure.h"
 #include "pressure/pressure_3d.h"
@@ -62,9 +64,10 @@ extern "C" int ref_pcg(int dim, int nx, int ny, int nz, double lx, double ly, do
     return 0;
 }
 
-// PCG with the preconditioner of a factory key (factory.cpp:19,23,
+// PCG with the preconditioner of a factory key (factory.cpp:19-23,
 // factory_3d.cpp:19): precond 0 = UAAMGPreconditioner[3D] ("pcg_uaamg"),
-// 1 = IdentityPreconditioner[3D] ("cg"); lap_diag/lap_off_x/lap_off_y
+// 1 = IdentityPreconditioner[3D] ("cg"), 2 = GMGPreconditioner ("pcg",
+// "pcg_gmg", 2D), 3 = AMGPreconditioner ("pcg_amg", 2D); lap_diag/lap_off_x/lap_off_y
 // (2D, all or none) fill Grid's variable Laplacian (grid.h:21-23,
 // init_variable_lap, grid.cpp) that PCG::solve's matvec reads (pcg.cpp:36-46).
 extern "C" int ref_pcg_ex(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
@@ -82,13 +85,15 @@ extern "C" int ref_pcg_ex(int dim, int nx, int ny, int nz, double lx, double ly,
         std::vector<double> r(rhs, rhs + N);
         std::unique_ptr<Preconditioner> pc;
         if (precond == 1) pc = std::make_unique<IdentityPreconditioner>();
+        else if (precond == 2) pc = std::make_unique<GMGPreconditioner>();
+        else if (precond == 3) pc = std::make_unique<AMGPreconditioner>();
         else pc = std::make_unique<UAAMGPreconditioner>();
         PCG s(std::move(pc));
         s.solve(g, r, max_iter, tol);
         for (size_t c = 0; c < N; c++) p[c] = g.p[c];
         return 0;
     }
-    if (lap_diag) return -1;
+    if (lap_diag || precond >= 2) return -1;
     Grid3D g(nx, ny, nz, lx, ly, lz);
     size_t N = g.p.size();
     for (size_t c = 0; c < N; c++) { g.solid[c] = solid[c] != 0; g.p[c] = p[c]; }

@@ -17,6 +17,10 @@
  *   PCG            src/solver/pcg.cpp:18-126, src/solver/pcg_3d.cpp:18-125
  *   relative tol   src/solver/cuda/cuda_pcg_3d_opt.cu:376-380,410
  *   amgpcg         test/solver/test_amgpcg.cpp:6-70, test/test_common.h:16-97
+ *   identity       include/solver/preconditioner/identity_preconditioner.h:19
+ *   GMG / AMG      src/solver/preconditioner/gmg_preconditioner.cpp:13-194,
+ *                  src/solver/preconditioner/amg_preconditioner.cpp:13-209
+ *   variable lap   src/solver/pcg.cpp:36-46 (Grid::lap_*, grid.h:21-23)
  *
  * Extensions beyond the reference, all inactive with default options and
  * Neumann walls (then every result is the reference's, bit for bit):
@@ -400,6 +404,166 @@ int orc_level_data(const orc_grid* g, const orc_opts* o, int level,
     return rc;
 }
 
+/* ---------------- GMG and AMG preconditioners (2D; keys "pcg"/"pcg_gmg", "pcg_amg") ----------------
+ * GMGPreconditioner (src/solver/preconditioner/gmg_preconditioner.cpp:13-194) and
+ * AMGPreconditioner (amg_preconditioner.cpp:13-209): the same hierarchy shape
+ * (:13-27 / :157-173), coarse solid iff at least half of the 2x2 children
+ * are (:29-39 / :141-155), a smoother that adds (b - diag x + lap) / d_eff
+ * in place per colour (:41-81 / :13-50), V(2,2) with 10 x (forward +
+ * reverse) sweeps on the coarsest level (:163-174 / :175-188; here
+ * coarsest_sweeps pairs, 10 by default), and
+ * restriction by the mean of the fluid children's residuals; AMG forms the
+ * residual as b - (diag x - lap) and accumulates it through its aggregate
+ * map (:91-111), GMG as (b - diag x) + lap per coarse cell (:121-142). */
+
+typedef struct {
+    int nx, ny;
+    double dx, dy, idx2, idy2, diag;
+    double *p, *b;
+    unsigned char* solid;
+    size_t N;
+} GLevel;
+
+typedef struct { int nl, amg, csweeps; GLevel L[MAXL]; } GHier;
+
+#define GX2(L, i, j) ((size_t)(i) + (size_t)(j) * (size_t)((L)->nx + 2))
+
+static void gmg_free(GHier* H) {
+    for (int l = 0; l < H->nl; l++) { free(H->L[l].p); free(H->L[l].b); free(H->L[l].solid); }
+    H->nl = 0;
+}
+
+static int gmg_build(GHier* H, const orc_grid* g, const orc_opts* o, int amg) {
+    int nx = g->nx, ny = g->ny;
+    double dx = g->dx, dy = g->dy;
+    H->nl = 0; H->amg = amg; H->csweeps = o->coarsest_sweeps;
+    while (nx >= 2 && ny >= 2 && H->nl < MAXL) {
+        GLevel* L = &H->L[H->nl++];
+        L->nx = nx; L->ny = ny; L->dx = dx; L->dy = dy;
+        L->idx2 = 1.0 / (dx * dx); L->idy2 = 1.0 / (dy * dy);
+        L->diag = amg ? 2.0 * L->idx2 + 2.0 * L->idy2 : 2.0 * (L->idx2 + L->idy2);
+        L->N = (size_t)(nx + 2) * (ny + 2);
+        L->p = calloc(L->N, sizeof(double)); L->b = calloc(L->N, sizeof(double)); L->solid = calloc(L->N, 1);
+        if (!L->p || !L->b || !L->solid) return -2;
+        if (nx <= o->coarsest_side || ny <= o->coarsest_side) break;
+        nx /= 2; ny /= 2; dx *= 2.0; dy *= 2.0;
+    }
+    return 0;
+}
+
+/* one colour ((i + j) & 1 == odd) of the in-place smoother */
+static void gmg_colour(GLevel* L, int odd) {
+    const unsigned char* S = L->solid;
+    const double idx2 = L->idx2, idy2 = L->idy2, diag = L->diag;
+    for (int i = 1; i <= L->nx; i++)
+        for (int j = 1 + ((i + odd + 1) % 2); j <= L->ny; j += 2) {
+            size_t id = GX2(L, i, j);
+            if (S[id]) continue;
+            double* P = L->p;
+            double pL = (i > 1 && !S[GX2(L, i - 1, j)]) ? P[GX2(L, i - 1, j)] : P[id];
+            double pR = (i < L->nx && !S[GX2(L, i + 1, j)]) ? P[GX2(L, i + 1, j)] : P[id];
+            double pB = (j > 1 && !S[GX2(L, i, j - 1)]) ? P[GX2(L, i, j - 1)] : P[id];
+            double pT = (j < L->ny && !S[GX2(L, i, j + 1)]) ? P[GX2(L, i, j + 1)] : P[id];
+            double lap = (pL + pR) * idx2 + (pB + pT) * idy2;
+            double eff_d = diag;
+            if (i == 1 || S[GX2(L, i - 1, j)]) eff_d -= idx2;
+            if (i == L->nx || S[GX2(L, i + 1, j)]) eff_d -= idx2;
+            if (j == 1 || S[GX2(L, i, j - 1)]) eff_d -= idy2;
+            if (j == L->ny || S[GX2(L, i, j + 1)]) eff_d -= idy2;
+            P[id] += ((eff_d < 1e-15) ? 0.0 : 1.0 / eff_d) * (L->b[id] - diag * P[id] + lap);
+        }
+}
+/* smooth: odd then even colour; smoothReverse: even then odd */
+static void gmg_smooth(GLevel* L, int sweeps, int reverse) {
+    for (int s = 0; s < sweeps; s++) { gmg_colour(L, reverse ? 0 : 1); gmg_colour(L, reverse ? 1 : 0); }
+}
+
+static double gmg_resid(const GLevel* F, int i, int j, int amg) {
+    const unsigned char* S = F->solid;
+    const double* P = F->p;
+    size_t id = GX2(F, i, j);
+    double pL = (i > 1 && !S[GX2(F, i - 1, j)]) ? P[GX2(F, i - 1, j)] : P[id];
+    double pR = (i < F->nx && !S[GX2(F, i + 1, j)]) ? P[GX2(F, i + 1, j)] : P[id];
+    double pB = (j > 1 && !S[GX2(F, i, j - 1)]) ? P[GX2(F, i, j - 1)] : P[id];
+    double pT = (j < F->ny && !S[GX2(F, i, j + 1)]) ? P[GX2(F, i, j + 1)] : P[id];
+    if (amg) {
+        double Ax = F->diag * P[id] - (pL + pR) * F->idx2 - (pB + pT) * F->idy2;
+        return F->b[id] - Ax;
+    }
+    double lap = (pL + pR) * F->idx2 + (pB + pT) * F->idy2;
+    return F->b[id] - F->diag * P[id] + lap;
+}
+
+/* mean of the fluid children's residuals (children in the order (0,0),(0,1),(1,0),(1,1)) */
+static void gmg_restrict(const GLevel* F, GLevel* C, int amg) {
+    for (int ic = 1; ic <= C->nx; ic++)
+        for (int jc = 1; jc <= C->ny; jc++) {
+            double sum = 0; int cnt = 0;
+            for (int di = 0; di < 2; di++)
+                for (int dj = 0; dj < 2; dj++) {
+                    int fi = 2 * ic - 1 + di, fj = 2 * jc - 1 + dj;
+                    if (F->solid[GX2(F, fi, fj)]) continue;
+                    sum += gmg_resid(F, fi, fj, amg);
+                    cnt++;
+                }
+            size_t cid = GX2(C, ic, jc);
+            if (amg) C->b[cid] = (cnt > 0) ? sum / cnt : 0.0;
+            else if (!C->solid[cid] && cnt > 0) C->b[cid] = sum / cnt;
+        }
+}
+
+/* GMG adds x_c to the fluid children of fluid coarse cells (gmg :144-156); AMG
+ * adds x_c of the aggregate to every fluid child (amg :113-123), which for a
+ * solid coarse cell is its zero-filled x */
+static void gmg_prolong(const GLevel* C, GLevel* F, int amg) {
+    for (int ic = 1; ic <= C->nx; ic++)
+        for (int jc = 1; jc <= C->ny; jc++) {
+            size_t cid = GX2(C, ic, jc);
+            if (C->solid[cid] && !amg) continue;
+            for (int di = 0; di < 2; di++)
+                for (int dj = 0; dj < 2; dj++) {
+                    size_t fid = GX2(F, 2 * ic - 1 + di, 2 * jc - 1 + dj);
+                    if (!F->solid[fid]) F->p[fid] += C->p[cid];
+                }
+        }
+}
+
+static void gmg_vcycle(GHier* H, int l) {
+    GLevel* L = &H->L[l];
+    if (l == H->nl - 1) {
+        for (int s = 0; s < H->csweeps; s++) { gmg_smooth(L, 1, 0); gmg_smooth(L, 1, 1); }
+        return;
+    }
+    gmg_smooth(L, 2, 0);
+    GLevel* C = &H->L[l + 1];
+    gmg_restrict(L, C, H->amg);
+    memset(C->p, 0, C->N * sizeof(double));
+    gmg_vcycle(H, l + 1);
+    gmg_prolong(C, L, H->amg);
+    gmg_smooth(L, 2, 1);
+}
+
+static void gmg_apply(GHier* H, const orc_grid* g, const double* r, double* z) {
+    GLevel* L0 = &H->L[0];
+    for (size_t c = 0; c < L0->N; c++) L0->solid[c] = g->solid[c] ? 1 : 0;
+    for (int l = 1; l < H->nl; l++) {   /* at least half of the 2x2 children solid */
+        GLevel *F = &H->L[l - 1], *C = &H->L[l];
+        for (int ic = 1; ic <= C->nx; ic++)
+            for (int jc = 1; jc <= C->ny; jc++) {
+                int sc = 0;
+                for (int di = 0; di < 2; di++)
+                    for (int dj = 0; dj < 2; dj++) sc += F->solid[GX2(F, 2 * ic - 1 + di, 2 * jc - 1 + dj)] ? 1 : 0;
+                C->solid[GX2(C, ic, jc)] = sc >= 2;
+            }
+    }
+    for (int i = 1; i <= L0->nx; i++)
+        for (int j = 1; j <= L0->ny; j++) L0->b[GX2(L0, i, j)] = r[GX2(L0, i, j)];
+    for (int l = 0; l < H->nl; l++) memset(H->L[l].p, 0, H->L[l].N * sizeof(double));
+    gmg_vcycle(H, 0);
+    for (int i = 1; i <= L0->nx; i++)
+        for (int j = 1; j <= L0->ny; j++) z[GX2(L0, i, j)] = L0->p[GX2(L0, i, j)];
+}
+
 /* ---------------- PCG (pcg.cpp / pcg_3d.cpp) ---------------- */
 
 typedef struct { int dim, nx, ny, nz; const unsigned char* s; } Dom;
@@ -465,6 +629,23 @@ static size_t padded_size(const orc_grid* g) {
     return (size_t)(g->nx + 2) * (g->ny + 2) * (g->dim == 3 ? (size_t)(g->nz + 2) : 1);
 }
 
+/* one application of the preconditioner that o->precond names (z zeroed first) */
+int orc_precond_apply(const orc_grid* g, const orc_opts* o, const double* r, double* z) {
+    orc_opts d; if (!o) { orc_default_opts(&d); o = &d; }
+    if (check_grid(g)) return -1;
+    const size_t N = padded_size(g);
+    if (o->precond == ORC_PRECOND_UAAMG) return orc_uaamg_apply(g, o, r, z);
+    if (o->precond == ORC_PRECOND_IDENTITY) { memcpy(z, r, N * sizeof(double)); return 0; }
+    if (o->precond != ORC_PRECOND_GMG && o->precond != ORC_PRECOND_AMG) return -1;
+    if (g->dim != 2) return -1;
+    GHier* G = calloc(1, sizeof(GHier));
+    if (!G) return -2;
+    int rc = gmg_build(G, g, o, o->precond == ORC_PRECOND_AMG);
+    if (rc == 0) { memset(z, 0, N * sizeof(double)); gmg_apply(G, g, r, z); }
+    gmg_free(G); free(G);
+    return rc;
+}
+
 int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* p,
             int max_iter, double tol, orc_result* res) {
     orc_opts dflt; if (!o) { orc_default_opts(&dflt); o = &dflt; }
@@ -480,11 +661,15 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
     double* pd = malloc(N * sizeof(double));
     double* Ap = calloc(N, sizeof(double));
     Hier* H = malloc(sizeof(Hier));
+    GHier* G = calloc(1, sizeof(GHier));
     orc_result rr = {0, 1, 0, 0.0, 0.0};
     int rc = 0;
-    if (!rhs || !r || !z || !pd || !Ap || !H) { rc = -2; goto out_nohier; }
+    const int gmg = o->precond == ORC_PRECOND_GMG || o->precond == ORC_PRECOND_AMG;
+    if (!rhs || !r || !z || !pd || !Ap || !H || !G) { rc = -2; goto out_nohier; }
+    if (gmg && g->dim != 2) { rc = -1; goto out_nohier; }
     rc = build_hier(H, g, o);
     if (rc) goto out_nohier;
+    if (gmg && (rc = gmg_build(G, g, o, o->precond == ORC_PRECOND_AMG)) != 0) goto out;
 
     memcpy(rhs, rhs_in, N * sizeof(double));
     if (project) {
@@ -498,7 +683,7 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
     memcpy(r, rhs, N * sizeof(double));
     if (project) subtract_mean(d, r);
     const int ident = o->precond == ORC_PRECOND_IDENTITY;   /* z = r: identity_preconditioner.h:19 */
-    if (ident) memcpy(z, r, N * sizeof(double)); else apply_hier(H, g, r, z);
+    if (ident) memcpy(z, r, N * sizeof(double)); else if (gmg) gmg_apply(G, g, r, z); else apply_hier(H, g, r, z);
     rr.applies++;
     if (project) subtract_mean(d, z);
     memcpy(pd, z, N * sizeof(double));
@@ -521,7 +706,7 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
         rr.rnorm = sqrt(rsnew);
         int conv = (o->tol_mode == 1) ? (rsnew < tolsq) : (sqrt(rsnew) < tol);
         if (conv) { rr.status = 0; rr.iters = it + 1; goto out; }
-        if (ident) memcpy(z, r, N * sizeof(double)); else apply_hier(H, g, r, z);
+        if (ident) memcpy(z, r, N * sizeof(double)); else if (gmg) gmg_apply(G, g, r, z); else apply_hier(H, g, r, z);
         rr.applies++;
         if (project) subtract_mean(d, z);
         double beta = dotf(d, r, z) / rsold;
@@ -532,7 +717,8 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs_in, double* 
 out:
     free_hier(H);
 out_nohier:
-    free(H); free(rhs); free(r); free(z); free(pd); free(Ap);
+    if (G) gmg_free(G);
+    free(G); free(H); free(rhs); free(r); free(z); free(pd); free(Ap);
     if (res) *res = rr;
     return rc;
 }

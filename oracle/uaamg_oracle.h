@@ -42,12 +42,16 @@ typedef struct {
     int coarsest_sweeps;         /* symmetric sweep pairs at the coarsest level (reference 10) */
     int tol_mode;                /* 0: sqrt(r.r) < tol (pcg.cpp:112); 1: r.r < tol^2 r0.r0 */
     int project_mean;            /* -1 auto (on iff every wall Neumann), 0 off, 1 on */
-    int precond;                 /* ORC_PRECOND_UAAMG (key "pcg_uaamg") or ORC_PRECOND_IDENTITY
-                                    (key "cg": IdentityPreconditioner, z = r; factory.cpp:19) */
+    int precond;                 /* ORC_PRECOND_UAAMG (key "pcg_uaamg"), ORC_PRECOND_IDENTITY
+                                    (key "cg": IdentityPreconditioner, z = r; factory.cpp:19),
+                                    ORC_PRECOND_GMG (keys "pcg", "pcg_gmg": GMGPreconditioner, 2D) or
+                                    ORC_PRECOND_AMG (key "pcg_amg": AMGPreconditioner, 2D) */
 } orc_opts;
 
 #define ORC_PRECOND_UAAMG    0
 #define ORC_PRECOND_IDENTITY 1
+#define ORC_PRECOND_GMG      2
+#define ORC_PRECOND_AMG      3
 
 typedef struct {
     int iters;       /* loop iterations entered (matvecs); = CudaPCG3D::last_iters when converged */
@@ -68,6 +72,12 @@ int orc_pcg(const orc_grid* g, const orc_opts* o, const double* rhs, double* p,
  * (uaamg_preconditioner.cpp:177-199, uaamg_preconditioner_3d.cpp:198-229).
  * Writes interior cells of z only. */
 int orc_uaamg_apply(const orc_grid* g, const orc_opts* o, const double* r, double* z);
+
+/* One application of the preconditioner o->precond names: UAAMG as above,
+ * identity z = r, or one GMG / AMG V(2,2) cycle on a hierarchy set up from
+ * scratch (GMGPreconditioner::apply, gmg_preconditioner.cpp:176-194;
+ * AMGPreconditioner::apply, amg_preconditioner.cpp:190-209; 2D only). */
+int orc_precond_apply(const orc_grid* g, const orc_opts* o, const double* r, double* z);
 
 /* Hierarchy shape: number of levels and (nx,ny,nz) per level (buildHierarchy). */
 int orc_hierarchy_dims(const orc_grid* g, const orc_opts* o, int* nlevels, int* dims, int max_levels);

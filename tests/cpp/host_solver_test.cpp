@@ -44,10 +44,13 @@ static int run_keys() {
     try { Factory3D::create("pcg_uaamg", Backend::CPU); } catch (const std::invalid_argument&) { threw = true; }
     CHECK(threw);
     threw = false;
-    try { Factory::create("pcg_gmg"); } catch (const std::invalid_argument&) { threw = true; }
-    CHECK(threw);
-    // "cg" and "pcg_uaamg" have HIP back-ends (contexts are created on the first solve)
+    try { Factory3D::create("pcg_gmg"); } catch (const std::invalid_argument&) { threw = true; }
+    CHECK(threw);   // the 3D factory has no GMG / AMG (factory_3d.cpp:21-23)
+    // the PCG keys have HIP back-ends (contexts are created on the first solve)
     CHECK(Factory::create("cg")->name() == "CG(HIP)");
+    CHECK(Factory::create("pcg")->name() == "PCG(HIP-GMG)");
+    CHECK(Factory::create("pcg_gmg")->name() == "PCG(HIP-GMG)");
+    CHECK(Factory::create("pcg_amg")->name() == "PCG(HIP-AMG)");
     CHECK(Factory3D::create("cg")->name() == "CG3D(HIP)");
     CHECK(Factory::create("pcg_uaamg")->name() == "PCG(HIP-UAAMG)");
     CHECK(Factory3D::create("pcg_uaamg")->name() == "PCG3D(HIP-UAAMG3D)");
@@ -113,6 +116,17 @@ static int run_solve(const std::string& out) {
         CHECK(hc && hc->last_status() == UAAMG_OK);
         std::printf("2d cg iters %d status %d\n", hc->last_stats().iters, hc->last_status());
         write_f64(out + "/g2_cg_p.f64", gc.p);
+        // keys "pcg_gmg" and "pcg_amg": the same system with the GMG / AMG preconditioners
+        for (const char* key : {"pcg_gmg", "pcg_amg"}) {
+            Grid gm(40, 24, 2.0, 1.2);
+            gm.solid = g.solid;
+            auto m = Factory::create(key, Backend::HIP, &o);
+            m->solve(gm, rhs, 300, 1e-10);
+            auto* hm = dynamic_cast<HipPCGSolver*>(m.get());
+            CHECK(hm && hm->last_status() == UAAMG_OK);
+            std::printf("2d %s iters %d status %d\n", key, hm->last_stats().iters, hm->last_status());
+            write_f64(out + "/g2_" + key + "_p.f64", gm.p);
+        }
         // variable Laplacian (grid.h:21-23): x couplings varying with y, y couplings with x
         Grid gv(40, 24, 2.0, 1.2);
         gv.solid = g.solid;

@@ -128,8 +128,27 @@ def g5_cases():
     return out
 
 
-def write_g5():
-    for name, pb, rhs, mi, rel, prec in g5_cases():
+def g6_cases():
+    """Keys "pcg" / "pcg_gmg" (GMGPreconditioner) and "pcg_amg"
+    (AMGPreconditioner), factory.cpp:20-22: 2D only."""
+    out = []
+    rng = np.random.default_rng(6)
+    for prec, tag in ((ob.PRECOND_GMG, "gmg"), (ob.PRECOND_AMG, "amg")):
+        pb = ob.Problem((48, 40), (1.2, 1.0)); pb.solid = cyl_mask(pb, 0.4, 0.5, 0.15)
+        out.append((f"g6_2d_48x40_cylinder_{tag}", pb, 1e3 * gaussian(pb), 300, 1e-6, prec))
+        pb = ob.Problem((33, 17), (1.0, 0.6))
+        r = np.zeros(pb.N); pb.interior(r)[...] = 1e3 * rng.standard_normal((17, 33))
+        out.append((f"g6_2d_33x17_odd_random_{tag}", pb, r, 300, 1e-8, prec))
+        # x couplings varying with y, y couplings with x, under the GMG / AMG V-cycle
+        pb = ob.Problem((48, 36), (1.2, 0.9)); pb.solid = cyl_mask(pb, 0.5, 0.45, 0.12)
+        pb.lap = shared_cases.separable_lap(pb)
+        out.append((f"g6_2d_48x36_separable_varlap_{tag}", pb, 1e3 * gaussian(pb), 600, 1e-6, prec))
+    return out
+
+
+def write_g5(groups=("g5",)):
+    todo = (g5_cases() if "g5" in groups else []) + (g6_cases() if "g6" in groups else [])
+    for name, pb, rhs, mi, rel, prec in todo:
         tol = rel * np.linalg.norm(rhs[pb.fluid_interior_mask()])    # absolute tolerance, relres rel
         p = ob.ref_pcg_ex(pb, rhs, mi, tol, precond=prec)
         d = dict(n=np.array(pb.n), L=np.array(pb.L), solid=pb.solid.copy(), rhs=rhs, p=p, max_iter=np.array(mi),
@@ -143,8 +162,8 @@ def write_g5():
 def main(groups):
     assert ob.ref_available(), "build oracle/_ref first: make -C oracle ref"
     os.makedirs(HERE, exist_ok=True)
-    if "g5" in groups:
-        write_g5()
+    if {"g5", "g6"} & set(groups):
+        write_g5(groups)
     if not ({"g1", "g2", "g3", "g4"} & set(groups)):
         return
     index = []
@@ -179,5 +198,5 @@ def main(groups):
 
 
 if __name__ == "__main__":
-    # python make_golden.py [g1 g2 g3 g4 g5]  (default: all groups)
-    main(sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5"])
+    # python make_golden.py [g1 g2 g3 g4 g5 g6]  (default: all groups)
+    main(sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6"])

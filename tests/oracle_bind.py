@@ -28,7 +28,7 @@ class OrcOpts(C.Structure):
                 ("precond", C.c_int)]
 
 
-PRECOND_UAAMG, PRECOND_IDENTITY = 0, 1
+PRECOND_UAAMG, PRECOND_IDENTITY, PRECOND_GMG, PRECOND_AMG = 0, 1, 2, 3
 
 
 class OrcResult(C.Structure):
@@ -48,6 +48,7 @@ def lib():
         _lib.orc_default_opts.argtypes = [P]
         _lib.orc_pcg.argtypes = [P, P, P, P, C.c_int, C.c_double, P]
         _lib.orc_uaamg_apply.argtypes = [P, P, P, P]
+        _lib.orc_precond_apply.argtypes = [P, P, P, P]
         _lib.orc_hierarchy_dims.argtypes = [P, P, P, P, C.c_int]
         _lib.orc_level_data.argtypes = [P, P, C.c_int, P, P, P, P, P, P]
         _lib.orc_matvec.argtypes = [P, P, P]
@@ -165,6 +166,17 @@ def uaamg_apply(pb, r, **opts):
     g = pb.grid()
     o = default_opts(**opts)
     rc = lib().orc_uaamg_apply(C.byref(g), C.byref(o), _p(r), _p(z))
+    assert rc == 0, rc
+    return z
+
+
+def precond_apply(pb, r, precond, **opts):
+    """One application of the preconditioner `precond` (PRECOND_*)."""
+    z = np.zeros(pb.N)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    g = pb.grid()
+    o = default_opts(precond=precond, **opts)
+    rc = lib().orc_precond_apply(C.byref(g), C.byref(o), _p(r), _p(z))
     assert rc == 0, rc
     return z
 

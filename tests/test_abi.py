@@ -124,7 +124,8 @@ def test_bad_arguments_rejected():
 
 @pytest.mark.parametrize("kw", [{"precision": U.MIXED_FP16}, {"precision": 6}, {"forms": 1 << 12},
                                 {"xthreads": 32}, {"xblocks": 0}, {"march_target": 0}, {"rows_kc": -4},
-                                {"tol_mode": 2}, {"nu_pre": -1}, {"preconditioner": 2}])
+                                {"tol_mode": 2}, {"nu_pre": -1}, {"preconditioner": 4},
+                                {"preconditioner": -1}])
 def test_invalid_options_are_argument_errors(kw):
     """Options are checked before the device: an unbuilt precision (FP16
     levels) or an out-of-range form or launch shape is UAAMG_ERR_ARG, not a
@@ -140,8 +141,9 @@ def test_invalid_options_are_argument_errors(kw):
     assert L.uaamg_server_create(C.byref(s), C.byref(o), 1 << 30, 4) == U.ERR_ARG
 
 
-@pytest.mark.parametrize("kw", [{"forms": U.FORM_DEFAULT & ~U.FORM_TAIL}, {"preconditioner": U.PRECOND_NONE}],
-                         ids=["no_tail", "identity"])
+@pytest.mark.parametrize("kw", [{"forms": U.FORM_DEFAULT & ~U.FORM_TAIL}, {"preconditioner": U.PRECOND_NONE},
+                                {"preconditioner": U.PRECOND_GMG}, {"preconditioner": U.PRECOND_AMG}],
+                         ids=["no_tail", "identity", "gmg", "amg"])
 def test_batch_and_server_need_uaamg_with_tail(kw):
     L = U.load().L
     o = U.load().default_opts(precision=U.FP32, **kw)
@@ -159,6 +161,21 @@ def test_bytes_required_mixed_f64_between_fp32_and_fp64():
     b = {p: L.uaamg_bytes_required(C.byref(g), C.byref(U.load().default_opts(precision=p)))
          for p in (U.FP64, U.MIXED_F64, U.FP32)}
     assert b[U.FP32] < b[U.MIXED_F64] < b[U.FP64]
+
+
+def test_bytes_required_gmg_amg_2d_only():
+    """GMG / AMG exist for 2D grids only (factory_3d.cpp:21-23): the byte
+    estimate of a 3D grid under them is 0 (uaamg_set_grid would refuse the
+    pair), a 2D grid's equals the UAAMG context's (same buffers)."""
+    L = U.load().L
+    g3 = capi.make_grid((32, 32, 32), (1 / 32,) * 3, np.zeros(34 ** 3, np.uint8))
+    g2 = capi.make_grid((128, 96), (1 / 128, 1 / 96), np.zeros(130 * 98, np.uint8))
+    for pc in (U.PRECOND_GMG, U.PRECOND_AMG):
+        o = U.load().default_opts(preconditioner=pc)
+        assert L.uaamg_bytes_required(C.byref(g3), C.byref(o)) == 0
+        assert L.uaamg_bytes_required(C.byref(g2), C.byref(o)) == \
+            L.uaamg_bytes_required(C.byref(g2), C.byref(U.load().default_opts()))
+    assert L.uaamg_bytes_required(C.byref(g3), C.byref(U.load().default_opts(preconditioner=4))) == 0
 
 
 def test_host_mirror_keys():

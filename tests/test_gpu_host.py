@@ -95,6 +95,18 @@ def test_host_mirror_cg_matches_oracle(host_run):
     assert _fluid_err(pb, _rd(d, "g2_cg_p.f64"), po) <= 1e-9
 
 
+@pytest.mark.parametrize("key,precond", [("pcg_gmg", ob.PRECOND_GMG), ("pcg_amg", ob.PRECOND_AMG)])
+def test_host_mirror_gmg_amg_match_oracle(host_run, key, precond):
+    """Factory::create("pcg_gmg") / ("pcg_amg"): PCG with the GMG / AMG
+    preconditioner (factory.cpp:21-22) equals the oracle's solve."""
+    d, _ = host_run
+    pb = ob.Problem((40, 24), (2.0, 1.2), solid=_rd(d, "g2_solid.u8", np.uint8))
+    rhs = _rd(d, "g2_rhs.f64")
+    po, res = ob.pcg(pb, rhs, 300, 1e-10, tol_mode=1, precond=precond)
+    assert res.status == 0
+    assert _fluid_err(pb, _rd(d, f"g2_{key}_p.f64"), po) <= 1e-9
+
+
 def test_host_mirror_variable_lap_matches_oracle(host_run):
     """A Grid with a variable Laplacian (init_variable_lap) solved through
     the factory's pcg_uaamg equals the oracle's PCG on the same coefficients."""

@@ -444,3 +444,125 @@ def test_cg_matches_dense_solve():
     x = np.linalg.lstsq(A, -rhs[idx], rcond=None)[0]
     a, b = p[idx] - p[idx].mean(), x - x.mean()
     assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max()
+
+
+# ---------------------------------------------------------------- keys "pcg" / "pcg_gmg" / "pcg_amg" (2D)
+
+GMG_AMG = [(ob.PRECOND_GMG, "gmg"), (ob.PRECOND_AMG, "amg")]
+
+
+@pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built (no reference tree here)")
+@pytest.mark.parametrize("precond,tag", GMG_AMG, ids=[t for _, t in GMG_AMG])
+@pytest.mark.parametrize("name,pb", [c for c in cases.neumann_cases() if c[1].dim == 2],
+                         ids=[c[0] for c in cases.neumann_cases() if c[1].dim == 2])
+def test_oracle_gmg_amg_match_reference_build(name, pb, precond, tag):
+    """PCG with GMGPreconditioner / AMGPreconditioner (factory.cpp:20-22),
+    bit for bit against the reference build: iterates, residual path and
+    iteration count."""
+    rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(7))
+    tol = 1e-8 * np.linalg.norm(rhs[pb.fluid_interior_mask()])
+    p1, res = ob.pcg(pb, rhs, 300, tol, precond=precond)
+    p2 = ob.ref_pcg_ex(pb, rhs, 300, tol, precond=precond)
+    assert res.status == 0 and res.iters > 3
+    assert np.array_equal(p1, p2)
+
+
+def _coarse_solid(pb):
+    """A coarse cell is solid when at least two of its 2x2 children are."""
+    nx, ny = pb.n
+    s = pb.view(pb.solid)[1:ny + 1, 1:nx + 1] != 0
+    cx, cy = nx // 2, ny // 2
+    cnt = s[:2 * cy, :2 * cx].reshape(cy, 2, cx, 2).sum(axis=(1, 3))
+    out = np.zeros((cy + 2, cx + 2), np.uint8)
+    out[1:-1, 1:-1] = cnt >= 2
+    return out.reshape(-1)
+
+
+def _dense_vcycle(pb, b, side=4, csweeps=10):
+    """The GMG / AMG V(2,2) cycle as dense linear algebra over the fluid
+    cells: red-black Gauss-Seidel on the graph Laplacian (odd colour i + j
+    first forward, even first in reverse), the coarse rhs the mean of the
+    fluid children's residuals b - A x, the coarse correction added to the
+    fluid children of fluid coarse cells, the coarse operator rediscretised
+    at twice the spacing."""
+    A, idx = dense_level0(pb)
+    sx = pb.n[0] + 2
+    par = (idx % sx + idx // sx) & 1
+    diag = np.diag(A).copy()
+
+    def colour(x, bb, odd):
+        for q in np.flatnonzero(par == odd):      # cells of one colour are decoupled
+            if diag[q] > 0:
+                x[q] += (bb[q] - A[q] @ x) / diag[q]
+
+    def sweep(x, bb, reverse):
+        colour(x, bb, 0 if reverse else 1)
+        colour(x, bb, 1 if reverse else 0)
+
+    bb = b[idx].copy()
+    x = np.zeros(idx.size)
+    if pb.n[0] <= side or pb.n[1] <= side or pb.n[0] // 2 < 2 or pb.n[1] // 2 < 2:
+        for _ in range(csweeps):
+            sweep(x, bb, False)
+            sweep(x, bb, True)
+        return idx, x
+    for _ in range(2):
+        sweep(x, bb, False)
+    res = np.zeros(pb.N); res[idx] = bb - A @ x
+    cn = (pb.n[0] // 2, pb.n[1] // 2)
+    pc = ob.Problem(cn, (2 * pb.h[0] * cn[0], 2 * pb.h[1] * cn[1]), _coarse_solid(pb))
+    fl = np.zeros(pb.N, bool); fl[idx] = True
+    bc = np.zeros(pc.N)
+    for jc in range(1, cn[1] + 1):
+        for ic in range(1, cn[0] + 1):
+            ch = [(2 * ic - 1 + di) + (2 * jc - 1 + dj) * sx for di in (0, 1) for dj in (0, 1)]
+            ch = [c for c in ch if fl[c]]
+            if ch:
+                bc[ic + jc * (cn[0] + 2)] = np.mean(res[ch])
+    cidx, xc = _dense_vcycle(pc, bc, side, csweeps)
+    xcf = np.zeros(pc.N); xcf[cidx] = xc
+    for q, c in enumerate(idx):
+        i, j = c % sx, c // sx
+        ic, jc = (i + 1) // 2, (j + 1) // 2
+        if ic <= cn[0] and jc <= cn[1]:
+            x[q] += xcf[ic + jc * (cn[0] + 2)]
+    for _ in range(2):
+        sweep(x, bb, True)
+    return idx, x
+
+
+@pytest.mark.parametrize("precond,tag", GMG_AMG, ids=[t for _, t in GMG_AMG])
+@pytest.mark.parametrize("n,L,block", [((8, 6), (1.0, 0.75), None), ((16, 12), (1.0, 0.6), (4, 9, 3, 7)),
+                                       ((13, 10), (1.3, 1.0), (2, 6, 5, 8)), ((20, 9), (2.0, 0.5), (7, 8, 1, 10))])
+def test_gmg_amg_vcycle_is_dense_multigrid(precond, tag, n, L, block):
+    """One GMG / AMG application equals the V(2,2) cycle written as dense
+    linear algebra (_dense_vcycle): a wrong colour order, stencil term,
+    restriction weight or prolongation target changes z.  The two classes
+    differ only in the order of their floating-point operations."""
+    pb = ob.Problem(n, L)
+    if block:
+        i0, i1, j0, j1 = block
+        pb.view(pb.solid)[j0:j1, i0:i1] = 1
+    r = cases.random_rhs(pb, np.random.default_rng(11))
+    z = ob.precond_apply(pb, r, precond)
+    idx, x = _dense_vcycle(pb, r)
+    assert np.abs(z[idx] - x).max() <= 1e-11 * np.abs(x).max()
+    # solid interior cells keep z = 0
+    s = np.zeros(pb.N, bool); pb.interior(s)[...] = pb.interior(pb.solid) != 0
+    assert not z[s].any()
+
+
+def test_gmg_and_amg_agree_to_rounding():
+    """GMG and AMG restate the same cycle (their residual and restriction
+    formulas are equal in exact arithmetic): their PCG solves agree to
+    rounding and take the same number of iterations within one."""
+    for name, pb in cases.neumann_cases():
+        if pb.dim != 2:
+            continue
+        rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(12))
+        pg, rg = ob.pcg(pb, rhs, 300, 1e-10, tol_mode=1, precond=ob.PRECOND_GMG)
+        pa, ra = ob.pcg(pb, rhs, 300, 1e-10, tol_mode=1, precond=ob.PRECOND_AMG)
+        m = pb.fluid_interior_mask()
+        a, b = pg[m] - pg[m].mean(), pa[m] - pa[m].mean()
+        assert rg.status == 0 and ra.status == 0 and abs(rg.iters - ra.iters) <= 1, name
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max(), name

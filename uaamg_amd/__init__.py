@@ -10,7 +10,8 @@ from .capi import (Lib, Context, Batch, Server, Stats, Opts, GridDesc, load, lib
                    NEUMANN, DIRICHLET, status_name, OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_NODEVICE,
                    ERR_ADMISSION, NOT_CONVERGED, BREAKDOWN, NONFINITE,
                    FORM_ROWS, FORM_ROWS_COARSE, FORM_TAIL, FORM_PMV, FORM_XMASK, FORM_NT_X, FORM_NT_P,
-                   FORM_NT_D, FORM_NT_U, FORM_SPLIT, FORM_NT, FORM_DEFAULT, PRECOND_UAAMG, PRECOND_NONE)
+                   FORM_NT_D, FORM_NT_U, FORM_SPLIT, FORM_NT, FORM_DEFAULT, PRECOND_UAAMG, PRECOND_NONE,
+                   PRECOND_GMG, PRECOND_AMG)
 from .build import build, build_host_tools
 
 __all__ = ["Lib", "Context", "Batch", "Server", "Stats", "Opts", "GridDesc", "load", "lib_path", "build", "build_host_tools",
@@ -18,4 +19,5 @@ __all__ = ["Lib", "Context", "Batch", "Server", "Stats", "Opts", "GridDesc", "lo
            "NEUMANN", "DIRICHLET", "status_name", "OK", "ERR_ARG", "ERR_HIP", "ERR_OOM", "ERR_NODEVICE",
            "ERR_ADMISSION", "NOT_CONVERGED", "BREAKDOWN", "NONFINITE",
            "FORM_ROWS", "FORM_ROWS_COARSE", "FORM_TAIL", "FORM_PMV", "FORM_XMASK", "FORM_NT_X", "FORM_NT_P",
-           "FORM_NT_D", "FORM_NT_U", "FORM_SPLIT", "FORM_NT", "FORM_DEFAULT", "PRECOND_UAAMG", "PRECOND_NONE"]
+           "FORM_NT_D", "FORM_NT_U", "FORM_SPLIT", "FORM_NT", "FORM_DEFAULT", "PRECOND_UAAMG", "PRECOND_NONE",
+           "PRECOND_GMG", "PRECOND_AMG"]

@@ -16,7 +16,7 @@ K_MATVEC, K_UPDATE, K_DOWN0, K_UP0, K_PUPDATE, K_COARSE, K_INIT, K_OTHER = range
 FORM_ROWS, FORM_ROWS_COARSE, FORM_TAIL, FORM_PMV, FORM_XMASK = 1, 2, 4, 8, 16
 FORM_NT_X, FORM_NT_P, FORM_NT_D, FORM_NT_U, FORM_SPLIT = 32, 64, 128, 256, 512
 FORM_NT = FORM_NT_X | FORM_NT_P | FORM_NT_D | FORM_NT_U
-PRECOND_UAAMG, PRECOND_NONE = 0, 1
+PRECOND_UAAMG, PRECOND_NONE, PRECOND_GMG, PRECOND_AMG = 0, 1, 2, 3
 FORM_DEFAULT = FORM_ROWS | FORM_ROWS_COARSE | FORM_TAIL | FORM_PMV | FORM_NT
 
 

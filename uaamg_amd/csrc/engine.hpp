@@ -20,6 +20,7 @@
 #include "kernels_tail.hpp"
 #include "kernels_pmv.hpp"
 #include "kernels_batch.hpp"
+#include "kernels_gmg.hpp"
 
 namespace uaamg {
 
@@ -189,7 +190,11 @@ public:
         for (int f = 0; f < 2 * D; f++) if (wall[f] != UAAMG_WALL_NEUMANN) all_neumann = 0;
         project_ = (o.project_mean < 0) ? all_neumann : o.project_mean;
         ident_ = o.preconditioner == UAAMG_PRECOND_NONE;
-        if (!ident_) rows_setup(h, wall);   // the identity form runs the flat kernels only
+        gmg_ = o.preconditioner == UAAMG_PRECOND_GMG ? 1 : (o.preconditioner == UAAMG_PRECOND_AMG ? 2 : 0);
+        if (gmg_ && (D != 2 || sizeof(S) < 4))
+            throw HipError{hipErrorInvalidValue, "GMG / AMG preconditioners: 2D grids, FP64 or FP32 levels"};
+        if (gmg_) gmg_setup();
+        if (!ident_ && !gmg_) rows_setup(h, wall);   // the identity and GMG / AMG forms run the flat kernels only
         timer_.s = stream_;
         if (pmv_on_) {   // the x update's stream, created before any capture
             HIPCHK(hipStreamCreateWithFlags(&xs_, hipStreamNonBlocking));
@@ -269,6 +274,11 @@ public:
             HIPCHK(hipGetLastError());
             return;
         }
+        if (gmg_) {
+            zsums_fused_ = false;
+            vcycle_gmg(r, z, skip);
+            return;
+        }
         zsums_fused_ = o_.fused && o_.nu_pre == 1 && o_.nu_post == 1 && fused_ok_;
         if (zsums_fused_) vcycle_fused(r, z, skip);
         else vcycle_pass(r, z, skip);
@@ -317,6 +327,98 @@ public:
         }
     }
 
+    // ---- GMG / AMG (keys "pcg", "pcg_gmg", "pcg_amg"; 2D) ----
+    // The same hierarchy shape and coarse solid masks as the UAAMG levels
+    // (gmg_preconditioner.cpp:13-39), a rediscretised operator per level
+    // (h doubles per level) and V(2,2) with coarsest_sweeps (forward +
+    // reverse) pairs on the coarsest level (:163-174).  Level 0 smooths z in
+    // place against r; the coarse levels use their own b and x.
+    std::vector<GmgC<double>> gmgc_;
+    void gmg_setup() {
+        gmgc_.clear();
+        double dx = h_[0], dy = h_[1];
+        for (size_t l = 0; l < shape_.size(); l++) {
+            GmgC<double> c;
+            c.idx2 = 1.0 / (dx * dx); c.idy2 = 1.0 / (dy * dy);
+            // the two classes' diagonals, each as its source writes it
+            // (gmg_preconditioner.cpp:43, amg_preconditioner.cpp:15)
+            c.diag = gmg_ == 2 ? 2.0 * c.idx2 + 2.0 * c.idy2 : 2.0 * (c.idx2 + c.idy2);
+            gmgc_.push_back(c);
+            dx *= 2.0; dy *= 2.0;
+        }
+    }
+    template <typename V>
+    GmgC<typename CTOf<V>::type> gmg_c(int l) const {
+        using X = typename CTOf<V>::type;
+        return GmgC<X>{(X)gmgc_[l].idx2, (X)gmgc_[l].idy2, (X)gmgc_[l].diag};
+    }
+    static dim3 gmg_grid(const Geo& g) { return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4)); }
+    template <typename V>
+    void gmg_colour(V* x, const V* b, const uint8_t* m, int l, int odd, const int* skip) {
+        const Geo& g = shape_[l];
+        k_gmg_colour<V><<<gmg_grid(g), dim3(64, 4), 0, stream_>>>(x, b, m, g, gmg_c<V>(l), odd, skip);
+        HIPCHK(hipGetLastError());
+    }
+    // smooth (odd colour first) or smoothReverse (even first), `sweeps` times
+    template <typename V>
+    void gmg_smooth(V* x, const V* b, const uint8_t* m, int l, int sweeps, bool reverse, const int* skip) {
+        for (int s = 0; s < sweeps; s++) {
+            gmg_colour<V>(x, b, m, l, reverse ? 0 : 1, skip);
+            gmg_colour<V>(x, b, m, l, reverse ? 1 : 0, skip);
+        }
+    }
+    template <typename VF>
+    void gmg_restrict(const VF* x, const VF* b, const uint8_t* m, int l, const int* skip) {
+        Level& C = lv_[l + 1];
+        auto kern = gmg_ == 2 ? k_gmg_restrict<true, VF, S> : k_gmg_restrict<false, VF, S>;
+        kern<<<gmg_grid(C.g), dim3(64, 4), 0, stream_>>>(x, b, m, shape_[l], gmg_c<VF>(l), C.b, C.mask, C.g, skip);
+        HIPCHK(hipGetLastError());
+    }
+    template <typename VF>
+    void gmg_prolong(VF* x, const uint8_t* m, int l, const int* skip) {
+        Level& C = lv_[l + 1];
+        auto kern = gmg_ == 2 ? k_gmg_prolong<true, VF, S> : k_gmg_prolong<false, VF, S>;
+        kern<<<gmg_grid(shape_[l]), dim3(64, 4), 0, stream_>>>(x, m, shape_[l], C.x, C.mask, C.g, skip);
+        HIPCHK(hipGetLastError());
+    }
+    void vcycle_gmg(const T* r, T* z, const int* skip) {
+        if constexpr (D == 2 && sizeof(S) >= 4) {
+            TScope ts(timer_, UAAMG_K_DOWN0);
+            const int nl = nlevels();
+            fill_async(z, g0_.N, skip);
+            for (int l = 1; l < nl; l++) fill_async(lv_[l].x, lv_[l].g.N, skip);
+            const int cs = std::max(0, o_.coarsest_sweeps);
+            if (nl == 1) {
+                for (int s = 0; s < cs; s++) {
+                    gmg_smooth<T>(z, r, solid0_, 0, 1, false, skip);
+                    gmg_smooth<T>(z, r, solid0_, 0, 1, true, skip);
+                }
+                return;
+            }
+            gmg_smooth<T>(z, r, solid0_, 0, 2, false, skip);
+            gmg_restrict<T>(z, r, solid0_, 0, skip);
+            for (int l = 1; l < nl - 1; l++) {
+                Level& L = lv_[l];
+                gmg_smooth<S>(L.x, L.b, L.mask, l, 2, false, skip);
+                gmg_restrict<S>(L.x, L.b, L.mask, l, skip);
+            }
+            {
+                Level& L = lv_[nl - 1];
+                for (int s = 0; s < cs; s++) {
+                    gmg_smooth<S>(L.x, L.b, L.mask, nl - 1, 1, false, skip);
+                    gmg_smooth<S>(L.x, L.b, L.mask, nl - 1, 1, true, skip);
+                }
+            }
+            for (int l = nl - 2; l >= 1; l--) {
+                Level& L = lv_[l];
+                gmg_prolong<S>(L.x, L.mask, l, skip);
+                gmg_smooth<S>(L.x, L.b, L.mask, l, 2, true, skip);
+            }
+            gmg_prolong<T>(z, solid0_, 0, skip);
+            gmg_smooth<T>(z, r, solid0_, 0, 2, true, skip);
+        }
+    }
+
     // ---- fused V-cycle: one down and one up launch per level, LDS coarsest ----
     // 2.5D marching kernels: in-plane region RX x RY (tile + 2-ring), PPT
     // positions per thread, march over KC planes of the slowest axis
@@ -347,7 +449,7 @@ public:
     }
     // ---- batch membership: this grid's buffers and launch shapes ----
     bool batch_ready() const {
-        return std::is_same<T, float>::value && !ident_ && fused_ok_ && tail_first_ > 0 && o_.nu_pre == 1 && o_.nu_post == 1
+        return std::is_same<T, float>::value && !ident_ && !gmg_ && fused_ok_ && tail_first_ > 0 && o_.nu_pre == 1 && o_.nu_post == 1
             && nlevels() <= BATCH_MAXL;
     }
     double alg_bytes_of(int iters) const { return alg_bytes(iters); }
@@ -993,6 +1095,14 @@ private:
         double l0 = 3 * sv + 2 * sl + 2 + 2 * rho * sl;
         double coarse = (5 * sl + 2 + 2 * ch * sc + 2 * rho * sl) * (rho / (1 - rho));
         double per_iter = ident_ ? cg + 2 * sv : cg + l0 + coarse;   // identity: z = r copy instead of the V-cycle
+        if (gmg_) {
+            // GMG / AMG V(2,2): per level 8 colour passes (x and mask read,
+            // half of b read and of x written), restriction (fine x, b, mask
+            // read, coarse b written), prolongation (fine x and mask read, x
+            // written, coarse x read), the x fill
+            auto lev = [&](double s, double sn) { return 8 * (2 * s + 1) + (2 * s + 1 + rho * sn) + (2 * s + 1 + rho * sn) + s; };
+            per_iter = cg + lev(sv, sl) + lev(sl, sl) * (rho / (1 - rho));
+        }
         if (vlap_.d) per_iter += 3 * 8;                              // the FP64 coefficient arrays of the operator
         double setup = 3 * sv + 3 + (1 + ch * sc) * (rho / (1 - rho));
         double init = 48;
@@ -1089,6 +1199,7 @@ private:
     bool trace() const { return o_.trace != 0; }   // relres history to stderr (cf. PCG_DBG, cuda_pcg_3d_opt.cu:384)
     int forms_used_ = 0;                           // UAAMG_FORM_* bits that ran in the current solve
     bool ident_ = false;                           // identity preconditioner (key "cg")
+    int gmg_ = 0;                                  // 1: GMG (keys "pcg", "pcg_gmg"), 2: AMG ("pcg_amg")
     VarLap vlap_{};                                // variable Laplacian (2D; d == nullptr: uniform)
     double* vlap_buf_ = nullptr;
     double* hist_ = nullptr; int hist_n_ = 0;

@@ -222,7 +222,7 @@ static bool opts_valid(const uaamg_opts* o, std::string& why) {
     if (o->forms & ~(UAAMG_FORM_DEFAULT | UAAMG_FORM_XMASK)) { why = "forms: unknown bits"; return false; }
     if (o->xfork_level < 0 || o->xblocks < 1 || o->xthreads < 64 || o->xthreads > 1024) { why = "x update launch"; return false; }
     if (o->rows_kc < 0 || o->march_kc_min < 0 || o->march_target < 1) { why = "rows_kc / march_*"; return false; }
-    if (o->preconditioner != UAAMG_PRECOND_UAAMG && o->preconditioner != UAAMG_PRECOND_NONE) { why = "preconditioner"; return false; }
+    if (o->preconditioner < UAAMG_PRECOND_UAAMG || o->preconditioner > UAAMG_PRECOND_AMG) { why = "preconditioner"; return false; }
     if (o->nu_pre < 0 || o->nu_post < 0 || o->nu_pre > 16 || o->nu_post > 16) { why = "nu_pre/nu_post"; return false; }
     if (o->coarsest_side < 1) { why = "coarsest_side"; return false; }
     if (o->coarsest_sweeps < 0) { why = "coarsest_sweeps"; return false; }
@@ -337,6 +337,8 @@ static int set_grid_impl(uaamg_ctx* c, const uaamg_grid* g, bool device_mask) {
     if (!c) return UAAMG_ERR_ARG;
     std::string why;
     if (!grid_valid(g, why)) return fail(c, UAAMG_ERR_ARG, why);
+    if (g->dim != 2 && (c->o.preconditioner == UAAMG_PRECOND_GMG || c->o.preconditioner == UAAMG_PRECOND_AMG))
+        return fail(c, UAAMG_ERR_ARG, "GMG / AMG preconditioners: 2D grids only (factory_3d.cpp:21-23)");
     CABI_TRY(c, {
         bool same = c->eng && c->dim == g->dim && c->eng_precision == c->o.precision && c->eng_side == c->o.coarsest_side;
         for (int q = 0; q < 3 && same; q++) same = (q < g->dim ? c->n[q] == g->n[q] && c->h[q] == g->h[q] : true);
@@ -587,6 +589,8 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
     uaamg_opts d;
     uaamg_default_opts(&d);
     if (!o) o = &d;
+    if (!opts_valid(o, why)) return 0;
+    if (g->dim != 2 && (o->preconditioner == UAAMG_PRECOND_GMG || o->preconditioner == UAAMG_PRECOND_AMG)) return 0;
     auto shape = hierarchy_shape(g->dim, g->n[0], g->n[1], g->dim == 3 ? g->n[2] : 1, o->coarsest_side);
     if (shape.empty()) return 0;
     // mirrors Engine::allocate (device bytes requested by a context)
