@@ -423,9 +423,10 @@ def bench_project(args):
 def bench_serve(args):
     """Served stream: 200 concurrent simulations submit one pressure solve per
     frame (relres 1e-5, warm start from their previous pressure).  Closed
-    loop first (every simulation always has a frame queued) to measure the
-    capacity, then Poisson arrivals at 75 % of it; latency = submit ->
-    completion.  Errors of sampled requests against an FP64 GPU solve of the
+    loop first (`slots` requests outstanding, each completion followed by a
+    frame of a uniformly drawn simulation) to measure the capacity, then
+    Poisson arrivals over uniformly drawn simulations at 75 % of it; latency
+    = submit -> completion, also per size class of the server.  Errors of sampled requests against an FP64 GPU solve of the
     same rhs (the FP64 path matches the FP64 oracle to 1e-9)."""
     import torch
     import uaamg_amd as U
@@ -458,20 +459,30 @@ def bench_serve(args):
         frames[k] += 1
         return req, r
 
-    # closed loop: warm-up pass over every simulation, then timed
+    # warm-up: the first (cold) frame of every simulation, drained
     live = {}
     for k in range(nsim):
         req, r = submit(k)
         live[req] = (k, r)
+    while live:
+        for d in srv.step():
+            live.pop(d, None)
     torch.cuda.synchronize()
+    # closed loop with the open loop's request mix: `slots` requests
+    # outstanding, every completion followed by a frame of a uniformly drawn
+    # simulation (a simulation drawn again while its frame runs queues behind
+    # it), so the capacity is that of the mix the arrivals bring
     t_closed = time.perf_counter()
     done_closed = 0
+    for _ in range(args.serve_slots):
+        req, r = submit(int(rng.integers(0, nsim)))
+        live[req] = r
     while time.perf_counter() - t_closed < args.serve_seconds:
         for d in srv.step():
-            k, _ = live.pop(d)
+            live.pop(d)
             done_closed += 1
-            req, r = submit(k)
-            live[req] = (k, r)
+            req, r = submit(int(rng.integers(0, nsim)))
+            live[req] = r
     cap = done_closed / (time.perf_counter() - t_closed)
     while live:
         for d in srv.step():
@@ -489,6 +500,7 @@ def bench_serve(args):
     live = {}
     lat = []
     iters = []
+    lat_class = {}   # size class (the server's: 2D / 3D, fewer / at least 2^18 padded cells) -> latencies
     sample = {}
     while nxt < nreq or live:
         now = time.perf_counter() - t_start
@@ -505,6 +517,8 @@ def bench_serve(args):
             st = srv.request(d)
             lat.append(st.ms_total)
             iters.append(st.iters)
+            cls = ("3d" if len(sims[k]["n"]) == 3 else "2d") + ("_large" if int(np.prod(sims[k]["shape"])) >= (1 << 18) else "_small")
+            lat_class.setdefault(cls, []).append(st.ms_total)
             if d in sample:
                 sample[d] = sample[d] + (p[k].clone(),)
                 torch.cuda.current_stream().synchronize()   # copied before the next frame of k can write p[k]
@@ -543,6 +557,8 @@ def bench_serve(args):
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "mean_iters": round(float(np.mean(iters)), 2),
+        "by_size_class": {c: {"requests": len(v), "p50_ms": round(float(np.percentile(v, 50)), 3),
+                              "p99_ms": round(float(np.percentile(v, 99)), 3)} for c, v in sorted(lat_class.items())},
         "hbm_budget_gb": args.serve_budget_gb,
         "slots": args.serve_slots,
         "max_rel_err_vs_fp64_sampled": max(errs) if errs else None,

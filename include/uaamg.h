@@ -284,10 +284,13 @@ int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, double* cons
  * Each request is one Solver::solve of one simulation frame
  * (include/solver/solver.h:29-30); the loop keeps each simulation's hierarchy
  * resident while the HBM budget allows (least recently used idle simulations
- * are evicted), runs every active request in one batch and admits new
- * requests between chunks of iterations (continuous batching).  With
- * warm_start = 1 the request's p_dev holds the previous frame's pressure and
- * receives the new one.  precision: UAAMG_FP32 or UAAMG_MIXED_BF16. */
+ * are evicted), runs the active requests in batches and admits new requests
+ * between chunks of iterations (continuous batching).  Simulations fall into
+ * four size classes (2D / 3D, fewer / at least 2^18 padded cells), each
+ * batched on a stream of its own, so small requests complete at the pace of
+ * their own chunks, not of the largest grid's.  With warm_start = 1 the
+ * request's p_dev holds the previous frame's pressure and receives the new
+ * one.  precision: UAAMG_FP32 or UAAMG_MIXED_BF16. */
 typedef struct uaamg_server uaamg_server;
 int uaamg_server_create(uaamg_server** out, const uaamg_opts* o, size_t hbm_budget, int max_active);
 void uaamg_server_destroy(uaamg_server* s);
@@ -301,9 +304,12 @@ int uaamg_server_register(uaamg_server* s, int sim, const uaamg_grid* g);
  * reports the request done. */
 int uaamg_server_submit(uaamg_server* s, int sim, const double* rhs_dev, double* p_dev, int warm_start,
                         int max_iter, double tol, int* req);
-/* Admit, run one chunk, complete: done[0..*n_done) = finished request ids.
- * Requests that finish while done[] is full are returned first by the next
- * step (their latency is taken when they finish). */
+/* Complete the size classes whose chunk of iterations has finished, admit
+ * queued requests into classes with no chunk running, launch the next chunk
+ * of every such class; if no class had finished, wait until one has (and
+ * refill it).  done[0..*n_done) = finished request ids.  Requests that finish
+ * while done[] is full are returned first by the next step (their latency is
+ * taken when they finish). */
 int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* n_done);
 /* Stats of a finished request (ms_total = latency from submit to completion). */
 int uaamg_server_request(const uaamg_server* s, int req, uaamg_stats* st);

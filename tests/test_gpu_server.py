@@ -104,7 +104,9 @@ def test_server_admission_under_budget(gpu):
     import torch
     small = [c for c in SIMS if c[1].dim == 3]
     import ctypes
-    o32 = U.load().default_opts(precision=U.FP32)
+    # a batch member's device bytes (members leave out the single-grid form's
+    # fused p-update buffers, UAAMG_FORM_PMV), as the server estimates them
+    o32 = U.load().default_opts(precision=U.FP32, forms=U.FORM_DEFAULT & ~U.FORM_PMV)
     est = max(U.load().L.uaamg_bytes_required(ctypes.byref(U.capi.make_grid(pb.n, pb.h, pb.solid, pb.wall)),
                                               ctypes.byref(o32)) for _, pb in small)
     budget = int(1.05 * est)      # the largest hierarchy fits only alone
@@ -148,4 +150,64 @@ def test_server_step_returns_every_completion(gpu):
     for r in reqs:
         st = srv.request(r)
         assert st is not None and st.status == U.OK and st.ms_total > 0, r
+    srv.close()
+
+
+def test_server_small_requests_do_not_wait_for_large_chunks(gpu):
+    """Size classes: a 3D grid of 2^18 or more padded cells runs in a batch
+    and on a stream of its own, so a small 2D simulation in a closed loop
+    (resubmitted as soon as it finishes) completes frames at the pace of its
+    own chunks while a 256^3 request runs.  Were both in one batch, every
+    small chunk would wait for a large one, and the small simulation could
+    finish at most (large chunks) / (chunks per small request) frames; the
+    test asks for twice that."""
+    import torch
+    from uaamg_amd import workloads as W
+    big = W.plume3d(seed=0, n=256)
+    name, small = [c for c in SIMS if c[0] == "2d_48x40_aniso_solids"][0]
+    srv = U.Server(1 << 34, max_active=8, precision=U.FP32, tol_mode=U.TOL_REL, use_graph=1)
+    srv.register(0, big.n, big.h, big.solid, big.wall)
+    srv.register(1, small.n, small.h, small.solid, small.wall)
+    rb, pbig = _dev(big.rhs), _dev(np.zeros(big.N))
+    rs = _dev(_rhs(small, 3))
+    psmall = [_dev(np.zeros(small.N)) for _ in range(2)]
+    req_big = srv.submit(0, rb.data_ptr(), pbig.data_ptr(), 0, 500, 1e-6)
+    frames, cur, last, last_req = 0, 0, None, None
+    pending = srv.submit(1, rs.data_ptr(), psmall[cur].data_ptr(), 0, 500, 1e-6)
+    big_done = False
+    t0 = time.time()
+    while not big_done or pending is not None:
+        for d in srv.step():
+            if d == req_big:
+                big_done = True
+            elif d == pending:
+                frames += 0 if big_done else 1
+                last, last_req, pending = cur, pending, None
+                if not big_done:   # the next frame of the small simulation, into the other buffer
+                    cur ^= 1
+                    psmall[cur].zero_()
+                    # torch's stream only: a device-wide synchronize would wait for the large chunk
+                    torch.cuda.current_stream().synchronize()
+                    pending = srv.submit(1, rs.data_ptr(), psmall[cur].data_ptr(), 0, 500, 1e-6)
+        assert time.time() - t0 < 60, "server did not finish"
+    torch.cuda.synchronize()
+    sb, ss = srv.request(req_big), srv.request(last_req)
+    lockstep = -(-sb.iters // 4) / -(-ss.iters // 4)   # chunks of 4 iterations (check_every)
+    print(f"large request {sb.iters} iterations, small {ss.iters}; small frames completed meanwhile: {frames} "
+          f"(one batch would allow {lockstep:.1f})", flush=True)
+    assert sb.status == 0 and ss.status == 0
+    assert frames >= 2 * lockstep, (frames, sb.iters, ss.iters)
+    c = U.Context(precision=U.FP32, tol_mode=U.TOL_REL)
+    c.set_grid(big.n, big.h, big.solid, big.wall)
+    p1, s1 = c.solve(big.rhs, 500, 1e-6)
+    c.close()
+    m = big.fluid_mask()
+    a, b = pbig.cpu().numpy()[m], p1[m]
+    a, b = a - a.mean(), b - b.mean()
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    c = U.Context(precision=U.FP32, tol_mode=U.TOL_REL)
+    c.set_grid(small.n, small.h, small.solid, small.wall)
+    p2, _ = c.solve(_rhs(small, 3), 500, 1e-6)
+    c.close()
+    assert _err(small, psmall[last].cpu().numpy(), p2) <= 1e-5
     srv.close()

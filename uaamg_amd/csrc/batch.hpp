@@ -65,7 +65,11 @@ public:
     BatchEngine& operator=(const BatchEngine&) = delete;
 
     int add(const int n[3], const double h[3], const int wall[6], const uint8_t* solid_dev) override {
-        auto e = std::make_unique<E>(o_, n, h, wall, s_);
+        // members run the batched kernels only: no fused p-update buffers or
+        // x-update stream of the single-grid form
+        uaamg_opts mo = o_;
+        mo.forms &= ~UAAMG_FORM_PMV;
+        auto e = std::make_unique<E>(mo, n, h, wall, s_);
         e->set_grid(solid_dev);
         e->set_idle();
         if (!e->batch_ready())

@@ -64,6 +64,11 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     double acc[1] = {0.0};
     const bool active = tile < rg.ntiles;   // idle workgroups still store a zero partial
     const int ty = tile % rg.nty, kc = tile / rg.nty;
+    // odd z-chunks march downwards, so the two tiles on either side of a
+    // chunk boundary read the planes they share at about the same time (both
+    // first or both last), and the second read can hit the L2: at 256^3 the
+    // pass fetches 195.9 instead of 199.3 MB (28.44 vs 28.54 ms per solve)
+    const int dir = (kc & 1) ? -1 : 1;
     RowThread<CPL, NW, RW, 1> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
@@ -92,9 +97,11 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
 #pragma unroll
             for (int q = 0; q < 3; q++) zs[q] += zp[(long)q * nzp + x];
     }
+    const int Pfirst = dir > 0 ? th.m0 - 1 : th.last + 1;
+    const int nst = th.last - th.m0 + 3;
     if (active) {
-        issue(0, th.m0 - 1);
-        issue(1, th.m0);
+        issue(0, Pfirst);
+        issue(1, Pfirst + dir);
     }
     for (int e = threadIdx.x; e < 64; e += NW * 64) sd[e] = K.d[e];
     if constexpr (FZ) {
@@ -145,13 +152,13 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     PN[S][t][q] = (nb & NB_FLUID)
                         ? fmaf(beta, pfp[PB][t][q], pfz[PB][t][q] - mean) : 0.0f;
                 }
-            issue(PB, P + 2);
+            issue(PB, P + 2 * dir);
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 L.ex[PB][w][0][q][lane] = PN[S][0][q];
                 L.ex[PB][w][1][q][lane] = PN[S][RW - 1][q];
             }
-            const int R = P - 1;
+            const int R = P - dir;
             if (R >= th.m0 && R <= th.last) {
                 constexpr int RB = S1 & 1;   // LDS buffer written when plane R entered
                 sfor<RW>([&](auto tc) {
@@ -205,14 +212,14 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
             }
             __syncthreads();
         };
-        for (int P = th.m0 - 1; P <= th.last + 1; P += 4) {   // m0 - 1 = 0 (mod 4)
-            step(IC<0>{}, P);
-            if (P + 1 > th.last + 1) break;
-            step(IC<1>{}, P + 1);
-            if (P + 2 > th.last + 1) break;
-            step(IC<2>{}, P + 2);
-            if (P + 3 > th.last + 1) break;
-            step(IC<3>{}, P + 3);
+        for (int k = 0; k < nst; k += 4) {
+            step(IC<0>{}, Pfirst + dir * k);
+            if (k + 1 >= nst) break;
+            step(IC<1>{}, Pfirst + dir * (k + 1));
+            if (k + 2 >= nst) break;
+            step(IC<2>{}, Pfirst + dir * (k + 2));
+            if (k + 3 >= nst) break;
+            step(IC<3>{}, Pfirst + dir * (k + 3));
         }
     }
     block_store<1>(acc, partials);

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <string>
 #include "../../include/uaamg.h"
 #include "kernels_proj.hpp"
@@ -902,13 +903,21 @@ extern "C" int uaamg_batch_solve(uaamg_batch* b, const double* const* rhs_dev, d
 // runs one chunk of iterations of every active request as one batch, and
 // completes the converged ones.  Requests that arrive while others run join
 // at the next chunk.
+//
+// Simulations are split into two size classes (below / from 2^18 padded
+// cells), each with a stream of its own and one batch per dimension: a class
+// whose chunk has finished is completed, refilled and relaunched while the
+// other runs, so a 64^2 request does not wait for the chunks of 256^3
+// members (a chunk of the large class takes milliseconds, one of the small
+// class tens of microseconds).  Two streams besides the caller's keep the
+// classes on hardware queues of their own (a process has four).
 
 namespace {
 struct SimState {
     int dim = 0, n[3] = {1, 1, 1}, wall[6] = {0, 0, 0, 0, 0, 0};
     double h[3] = {1, 1, 1};
     std::vector<uint8_t> mask;
-    int part = -1, slot = -1;          // resident: batch part and member index
+    int part = -1, slot = -1;          // resident: batch part (size class) and member index
     size_t bytes = 0;
     long last_used = 0;
     bool busy = false;
@@ -925,14 +934,26 @@ struct Request {
 };
 }  // namespace
 
+// batch part of a simulation: 2D small, 2D large, 3D small, 3D large; the
+// size class (small 0, large 1) is the part's low bit
+constexpr int kServeParts = 4;
+constexpr long kServeLarge = 1L << 18;   // padded cells from which a grid is large
+static int serve_part_of(int dim, long cells) { return (dim == 3 ? 2 : 0) + (cells >= kServeLarge ? 1 : 0); }
+static int serve_class_of(int part) { return part & 1; }
+
 struct uaamg_server {
     uaamg_opts o{};
     std::string err;
-    hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     int max_active = 0;
-    std::unique_ptr<uaamg::BatchBase> part[2];
-    std::vector<int> slot_sim[2];
+    // one batch per part; one stream and completion event per size class;
+    // inflight: a chunk of the class was launched and its event has not been
+    // seen complete
+    std::unique_ptr<uaamg::BatchBase> part[kServeParts];
+    hipStream_t cstream[2] = {};
+    hipEvent_t cev[2] = {};
+    bool inflight[2] = {};
+    std::vector<int> slot_sim[kServeParts];
     std::unordered_map<int, SimState> sims;
     std::vector<Request> reqs;
     std::deque<int> queue;
@@ -974,18 +995,28 @@ extern "C" int uaamg_server_create(uaamg_server** out, const uaamg_opts* o, size
     s->o = *o;
     s->budget = hbm_budget;
     s->max_active = max_active;
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { delete s; return UAAMG_ERR_HIP; }
+    // the class streams are created together (default priority), so that
+    // they land on different hardware queues
+    for (int c = 0; c < 2; c++)
+        if (hipStreamCreateWithFlags(&s->cstream[c], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->cev[c], hipEventDisableTiming) != hipSuccess) {
+            uaamg_server_destroy(s);
+            return UAAMG_ERR_HIP;
+        }
     *out = s;
     return UAAMG_OK;
 }
 
 extern "C" void uaamg_server_destroy(uaamg_server* s) {
     if (!s) return;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    s->part[0].reset();
-    s->part[1].reset();
+    for (int c = 0; c < 2; c++)
+        if (s->cstream[c]) (void)hipStreamSynchronize(s->cstream[c]);
+    for (int pi = 0; pi < kServeParts; pi++) s->part[pi].reset();
     if (s->stage) (void)hipFree(s->stage);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    for (int c = 0; c < 2; c++) {
+        if (s->cev[c]) (void)hipEventDestroy(s->cev[c]);
+        if (s->cstream[c]) (void)hipStreamDestroy(s->cstream[c]);
+    }
     delete s;
 }
 
@@ -1033,16 +1064,19 @@ static size_t sim_estimate(const uaamg_server* s, const SimState& st) {
     for (int q = 0; q < 3; q++) { g.n[q] = st.n[q]; g.h[q] = st.h[q]; }
     g.solid = st.mask.data();
     for (int f = 0; f < 6; f++) g.wall[f] = st.wall[f];
-    return uaamg_bytes_required(&g, &s->o);
+    uaamg_opts mo = s->o;
+    mo.forms &= ~UAAMG_FORM_PMV;   // as a batch member is built (BatchEngine::add)
+    return uaamg_bytes_required(&g, &mo);
 }
 
-// drop the least recently used idle resident simulation; false if none
+// drop the least recently used idle resident simulation of a class with no
+// chunk in flight; false if none
 static bool server_evict_one(uaamg_server* s) {
     int victim = -1;
     long best = 0;
     for (auto& kv : s->sims) {
         const SimState& st = kv.second;
-        if (st.slot < 0 || st.busy) continue;
+        if (st.slot < 0 || st.busy || s->inflight[serve_class_of(st.part)]) continue;
         if (victim < 0 || st.last_used < best) { victim = kv.first; best = st.last_used; }
     }
     if (victim < 0) return false;
@@ -1056,85 +1090,128 @@ static bool server_evict_one(uaamg_server* s) {
     return true;
 }
 
+// completions of the classes whose chunk has finished; with wait, blocks until
+// at least one in-flight class has finished (if any is in flight).  Returns
+// whether a class finished.
+static bool server_collect(uaamg_server* s, bool wait, std::vector<int>& fin) {
+    bool any = false;
+    for (;;) {
+        bool inflight = false;
+        for (int cl = 0; cl < 2; cl++) {
+            if (!s->inflight[cl]) continue;
+            inflight = true;
+            const hipError_t q = hipEventQuery(s->cev[cl]);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) throw HipError{q, "hipEventQuery"};
+            s->inflight[cl] = false;
+            any = true;
+            std::vector<int> still, mine;
+            for (int id : s->active) {
+                Request& r = s->reqs[id];
+                SimState& st = s->sims[r.sim];
+                if (serve_class_of(st.part) != cl) { still.push_back(id); continue; }
+                const uaamg::Ctrl& c = s->part[st.part]->member_ctrl(st.slot);
+                if (!c.done) { still.push_back(id); continue; }
+                s->part[st.part]->finish_member(st.slot, r.p, r.warm);
+                r.st.iters = c.iter;
+                r.st.status = c.status == uaamg::ST_CONVERGED ? UAAMG_OK
+                            : c.status == uaamg::ST_BREAKDOWN ? UAAMG_BREAKDOWN
+                            : c.status == uaamg::ST_NONFINITE ? UAAMG_NONFINITE : UAAMG_NOT_CONVERGED;
+                r.st.rnorm = std::sqrt(std::max(0.0, c.rsnew));
+                r.st.r0norm = std::sqrt(std::max(0.0, c.r0sq));
+                r.state = 2;
+                st.busy = false;
+                st.last_used = ++s->tick;
+                mine.push_back(id);
+            }
+            s->active.swap(still);
+            if (!mine.empty()) HIPCHK(hipStreamSynchronize(s->cstream[cl]));   // p written
+            const auto now = std::chrono::steady_clock::now();
+            for (int id : mine) {   // latency taken when the result is in p
+                Request& r = s->reqs[id];
+                r.t_done = now;
+                r.st.ms_total = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+                fin.push_back(id);
+            }
+        }
+        if (any || !wait || !inflight) return any;
+        std::this_thread::yield();
+    }
+}
+
+// admission, in arrival order, into classes with no chunk in flight; new
+// members join their batch before any starts; then one chunk of every idle
+// class with active members is launched on that class's stream
+static void server_admit_and_launch(uaamg_server* s) {
+    std::vector<int> starting;
+    for (auto it = s->queue.begin(); it != s->queue.end() && (int)(s->active.size() + starting.size()) < s->max_active;) {
+        Request& r = s->reqs[*it];
+        SimState& st = s->sims[r.sim];
+        if (st.busy) { ++it; continue; }              // its previous frame is still running
+        const int pi = st.slot >= 0 ? st.part : serve_part_of(st.dim, (long)st.mask.size());
+        const int cl = serve_class_of(pi);
+        if (s->inflight[cl]) { ++it; continue; }      // its class is running a chunk
+        if (st.slot < 0) {
+            const size_t need = sim_estimate(s, st);
+            while (s->used + need > s->budget && server_evict_one(s)) {}
+            if (s->used + need > s->budget) { ++it; continue; }   // waits for memory
+            if (!s->part[pi]) s->part[pi] = make_batch_part(st.dim, s->o, s->cstream[cl]);
+            const long N = (long)st.mask.size();
+            if (s->stage_n < N) {
+                if (s->stage) HIPCHK(hipFree(s->stage));
+                s->stage = nullptr;
+                HIPCHK(hipMalloc(&s->stage, N));
+                s->stage_n = N;
+            }
+            HIPCHK(hipMemcpyAsync(s->stage, st.mask.data(), N, hipMemcpyHostToDevice, s->cstream[cl]));
+            st.slot = s->part[pi]->add(st.n, st.h, st.wall, s->stage);
+            HIPCHK(hipStreamSynchronize(s->cstream[cl]));   // the staged mask is read before its next use
+            st.part = pi;
+            s->slot_sim[pi].push_back(r.sim);
+            st.bytes = s->part[pi]->member_bytes(st.slot);
+            s->used += st.bytes;
+        }
+        st.busy = true;
+        st.last_used = ++s->tick;
+        r.state = 1;
+        starting.push_back(*it);
+        it = s->queue.erase(it);
+    }
+    for (int pi = 0; pi < kServeParts; pi++)
+        if (s->part[pi] && !s->inflight[serve_class_of(pi)]) s->part[pi]->prepare();
+    for (int id : starting) {
+        Request& r = s->reqs[id];
+        SimState& st = s->sims[r.sim];
+        s->part[st.part]->start_member(st.slot, r.rhs, r.p, r.warm, r.max_iter, r.tol);
+        s->active.push_back(id);
+    }
+    bool run[kServeParts] = {};
+    for (int id : s->active) run[s->sims[s->reqs[id].sim].part] = true;
+    for (int cl = 0; cl < 2; cl++) {
+        if (s->inflight[cl]) continue;
+        bool any = false;
+        for (int pi = cl; pi < kServeParts; pi += 2)
+            if (run[pi]) { s->part[pi]->run_chunk(); any = true; }
+        if (!any) continue;
+        HIPCHK(hipEventRecord(s->cev[cl], s->cstream[cl]));
+        s->inflight[cl] = true;
+    }
+}
+
 extern "C" int uaamg_server_step(uaamg_server* s, int* done, int max_done, int* n_done) {
     if (!s) return UAAMG_ERR_ARG;
     if (n_done) *n_done = 0;
     SERVER_TRY(s, {
-        // admission, in arrival order; new members join their batch before any start
-        std::vector<int> starting;
-        for (auto it = s->queue.begin(); it != s->queue.end() && (int)(s->active.size() + starting.size()) < s->max_active;) {
-            Request& r = s->reqs[*it];
-            SimState& st = s->sims[r.sim];
-            if (st.busy) { ++it; continue; }              // its previous frame is still running
-            if (st.slot < 0) {
-                const size_t need = sim_estimate(s, st);
-                while (s->used + need > s->budget && server_evict_one(s)) {}
-                if (s->used + need > s->budget) { ++it; continue; }   // waits for memory
-                const int pi = st.dim == 3 ? 1 : 0;
-                if (!s->part[pi]) s->part[pi] = make_batch_part(st.dim, s->o, s->stream);
-                const long N = (long)st.mask.size();
-                if (s->stage_n < N) {
-                    if (s->stage) HIPCHK(hipFree(s->stage));
-                    s->stage = nullptr;
-                    HIPCHK(hipMalloc(&s->stage, N));
-                    s->stage_n = N;
-                }
-                HIPCHK(hipMemcpyAsync(s->stage, st.mask.data(), N, hipMemcpyHostToDevice, s->stream));
-                st.slot = s->part[pi]->add(st.n, st.h, st.wall, s->stage);
-                st.part = pi;
-                s->slot_sim[pi].push_back(r.sim);
-                st.bytes = s->part[pi]->member_bytes(st.slot);
-                s->used += st.bytes;
-            }
-            st.busy = true;
-            st.last_used = ++s->tick;
-            r.state = 1;
-            starting.push_back(*it);
-            it = s->queue.erase(it);
-        }
-        for (auto& p : s->part) if (p) p->prepare();
-        for (int id : starting) {
-            Request& r = s->reqs[id];
-            SimState& st = s->sims[r.sim];
-            s->part[st.part]->start_member(st.slot, r.rhs, r.p, r.warm, r.max_iter, r.tol);
-            s->active.push_back(id);
-        }
+        std::vector<int> fin;
+        // finished classes first (no wait), then admissions and launches;
+        // if no class had finished, wait for one and refill it at once
+        const bool got = server_collect(s, false, fin);
+        server_admit_and_launch(s);
+        if (!got && server_collect(s, true, fin)) server_admit_and_launch(s);
         // finished requests a previous step could not return (its done[] was full) come first
         int nd = 0;
         while (!s->undelivered.empty() && nd < max_done && done) { done[nd++] = s->undelivered.front(); s->undelivered.pop_front(); }
-        if (n_done) *n_done = nd;
-        if (s->active.empty()) return UAAMG_OK;
-        // one chunk of iterations for every active request, per batch part
-        bool run[2] = {false, false};
-        for (int id : s->active) run[s->sims[s->reqs[id].sim].part] = true;
-        for (int pi = 0; pi < 2; pi++) if (run[pi]) s->part[pi]->run_chunk();
-        HIPCHK(hipStreamSynchronize(s->stream));
-        // completions
-        std::vector<int> still, fin;
-        for (int id : s->active) {
-            Request& r = s->reqs[id];
-            SimState& st = s->sims[r.sim];
-            const uaamg::Ctrl& c = s->part[st.part]->member_ctrl(st.slot);
-            if (!c.done) { still.push_back(id); continue; }
-            s->part[st.part]->finish_member(st.slot, r.p, r.warm);
-            r.st.iters = c.iter;
-            r.st.status = c.status == uaamg::ST_CONVERGED ? UAAMG_OK
-                        : c.status == uaamg::ST_BREAKDOWN ? UAAMG_BREAKDOWN
-                        : c.status == uaamg::ST_NONFINITE ? UAAMG_NONFINITE : UAAMG_NOT_CONVERGED;
-            r.st.rnorm = std::sqrt(std::max(0.0, c.rsnew));
-            r.st.r0norm = std::sqrt(std::max(0.0, c.r0sq));
-            r.state = 2;
-            st.busy = false;
-            st.last_used = ++s->tick;
-            fin.push_back(id);
-        }
-        s->active.swap(still);
-        if (!fin.empty()) HIPCHK(hipStreamSynchronize(s->stream));
-        const auto now = std::chrono::steady_clock::now();
-        for (int id : fin) {   // every finished request gets its latency, returned now or by a later step
-            Request& r = s->reqs[id];
-            r.t_done = now;
-            r.st.ms_total = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+        for (int id : fin) {
             if (done && nd < max_done) done[nd++] = id;
             else s->undelivered.push_back(id);
         }
