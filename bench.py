@@ -239,6 +239,8 @@ def bench_batch(args):
             c.close()
         return dt, outs, iters
 
+    tuned = {}
+
     def batched(prec, reps):
         b = U.Batch(precision=prec, tol_mode=U.TOL_REL, use_graph=1, max_iter=1000)
         for p in pbs:
@@ -255,6 +257,7 @@ def bench_batch(args):
             rc, st = b.solve_device(R, P, None, 1000, tol)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / reps
+        tuned[prec] = {f"{d}d": b.tuning(d) for d in (2, 3)}
         b.close()
         return dt, outs, [s.iters for s in st], rc
 
@@ -311,6 +314,7 @@ def bench_batch(args):
         "iters_max": {"plain_fp32": int(max(it_plain)), "batched_fp32": int(max(it_b32)),
                       "batched_bf16": int(max(it_bbf))},
         "status": {"batched_fp32": U.status_name(rc32), "batched_bf16": U.status_name(rcbf)},
+        "autotuned": {"fp32": tuned.get(U.FP32), "bf16": tuned.get(U.MIXED_BF16)},
         "hierarchies": "resident (built before timing)",
     }
     log(f"[batch] {json.dumps(res)}")

@@ -272,6 +272,12 @@ int uaamg_batch_add_device(uaamg_batch* b, const uaamg_grid* g, int* id);
 int uaamg_batch_clear(uaamg_batch* b);
 int uaamg_batch_size(const uaamg_batch* b);
 size_t uaamg_batch_bytes(const uaamg_batch* b);
+/* Launch shapes the autotuner chose for the batch's 2D or 3D members (set by
+ * their first solve): march work-list density (workgroups aimed at per level
+ * and grid), fewest planes per march chunk, planes per level-0 row tile (0:
+ * the engine's choice).  No counterpart in the reference (it has no batched
+ * form).  UAAMG_ERR_ARG when the batch has no member of that dimension. */
+int uaamg_batch_tuning(const uaamg_batch* b, int dim, int* march_target, int* march_kc_min, int* rows_kc);
 /* Solve every member: rhs_dev[i], p_dev[i] are padded FP64 device arrays of
  * member i; warm_start[i] as in uaamg_solve_device (NULL: all cold).  st
  * (NULL or an array of uaamg_batch_size entries) receives per-member stats;

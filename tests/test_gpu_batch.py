@@ -42,6 +42,12 @@ def _solve_batch(members, prec, tol=1e-6, warm=None, p0=None, use_graph=1, rhs=N
     rc, st = b.solve_device([t.data_ptr() for t in rh], [t.data_ptr() for t in pd], warm, 500, tol)
     torch.cuda.synchronize()
     out = [t.cpu().numpy() for t in pd]
+    # the autotuner's choice is one of its candidates (coordinate descent
+    # over march density, march chunk and level-0 row tile)
+    for d in {pb.dim for _, pb in members}:
+        t = b.tuning(d)
+        assert t is not None and t["march_target"] in (64, 256, 1024) and t["march_kc_min"] in (2, 4, 8), t
+        assert t["rows_kc"] in (0, 8, 16), t
     b.close()
     return rc, st, out
 

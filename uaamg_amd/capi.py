@@ -52,6 +52,7 @@ EXPORTS = ["uaamg_default_opts", "uaamg_create", "uaamg_destroy", "uaamg_last_er
            "uaamg_amgpcg", "uaamg_amgpcg_noalloc", "uaamg_bytes_required", "uaamg_project", "uaamg_project_device",
            "uaamg_batch_create", "uaamg_batch_destroy", "uaamg_batch_last_error", "uaamg_batch_add",
            "uaamg_batch_add_device", "uaamg_batch_clear", "uaamg_batch_size", "uaamg_batch_bytes",
+           "uaamg_batch_tuning",
            "uaamg_batch_solve",
            "uaamg_server_create", "uaamg_server_destroy", "uaamg_server_last_error", "uaamg_server_register",
            "uaamg_server_submit", "uaamg_server_step", "uaamg_server_request", "uaamg_server_status"]
@@ -102,6 +103,7 @@ class Lib:
         L.uaamg_batch_size.argtypes = [P]
         L.uaamg_batch_bytes.argtypes = [P]
         L.uaamg_batch_bytes.restype = C.c_size_t
+        L.uaamg_batch_tuning.argtypes = [P, I, P, P, P]
         L.uaamg_batch_solve.argtypes = [P, P, P, P, I, D, P]
         L.uaamg_server_create.argtypes = [C.POINTER(C.c_void_p), P, C.c_size_t, I]
         L.uaamg_server_destroy.argtypes = [P]
@@ -369,6 +371,14 @@ class Batch:
 
     def bytes(self):
         return self.lib.L.uaamg_batch_bytes(self.h)
+
+    def tuning(self, dim):
+        """Launch shapes the autotuner chose for the dim-D members:
+        {"march_target", "march_kc_min", "rows_kc"}, or None without such members."""
+        t, k, r = C.c_int(0), C.c_int(0), C.c_int(0)
+        if self.lib.L.uaamg_batch_tuning(self.h, int(dim), C.byref(t), C.byref(k), C.byref(r)) != OK:
+            return None
+        return {"march_target": t.value, "march_kc_min": k.value, "rows_kc": r.value}
 
     def solve_device(self, rhs_ptrs, p_ptrs, warm=None, max_iter=-1, tol=-1.0):
         """rhs_ptrs / p_ptrs: device pointers (ints) of every member, in order.

@@ -38,6 +38,8 @@ public:
     virtual void finish() = 0;
     virtual void stats(int g, uaamg_stats* st, double ms_total) const = 0;
     virtual int tuned_target() const = 0;
+    virtual int tuned_kc_min() const = 0;
+    virtual int tuned_rows_kc() const = 0;
     virtual void prepare() = 0;
     virtual void start_member(int g, const double* rhs, double* pout, int warm, int max_iter, double tol) = 0;
     virtual void finish_member(int g, double* pout, int warm) = 0;
@@ -56,7 +58,8 @@ class BatchEngine final : public BatchBase {
     static constexpr int NLMAX = E::TAIL_NLMAX;
 
 public:
-    BatchEngine(const uaamg_opts& o, hipStream_t s) : o_(o), s_(s) {}
+    BatchEngine(const uaamg_opts& o, hipStream_t s)
+        : o_(o), s_(s), kc_min_(std::max(2, o.march_kc_min)), rows_kc_(std::max(0, o.rows_kc)) {}
     ~BatchEngine() {
         destroy_graph();
         free_all();
@@ -183,6 +186,8 @@ public:
         st->alg_bytes = eng_[g]->alg_bytes_of(c.iter);
     }
     int tuned_target() const override { return target_; }
+    int tuned_kc_min() const override { return kc_min_; }
+    int tuned_rows_kc() const override { return rows_kc_; }
 
     // ---- continuous batching: members start and finish on their own ----
     // (used by the server: members join between chunks, converged members
@@ -287,12 +292,14 @@ private:
             for (long b = 0; b < nb; b++) flat.push_back(BItem{g, (int)b, (int)nb, 0});
             // level 0
             if (D == 3 && d.cpl) {
+                // row tiles of rows_kc_ planes (0: the engine's own choice)
+                d.rgeo = E::make_rowgeo(d.g0.ny, d.g0.nz, E::ROWS_NW * E::ROWS_RW - 4, rows_kc_);
                 std::vector<BItem>& L = rows[d.cpl == 2 ? 0 : 1];
                 d.up_base = (int)L.size();
                 d.up_cnt = d.rgeo.ntiles;
                 for (int t = 0; t < d.rgeo.ntiles; t++) L.push_back(BItem{g, t, 0, 0});
             } else {
-                typename E::MLaunch m = E::march_launch(d.g0, RX0, RY0, target);
+                typename E::MLaunch m = E::march_launch(d.g0, RX0, RY0, target, kc_min_);
                 d.lKC[0] = m.KC;
                 d.up_base = (int)m0.size();
                 d.up_cnt = (int)(m.grid.x * m.grid.y * m.grid.z);
@@ -303,7 +310,7 @@ private:
             // march levels 1 .. lt-1
             for (int l = 1; l < d.lt; l++) {
                 if ((int)cl.size() <= l) cl.resize(l + 1);
-                typename E::MLaunch m = E::march_launch(d.lg[l], RXC, RYC, target);
+                typename E::MLaunch m = E::march_launch(d.lg[l], RXC, RYC, target, kc_min_);
                 d.lKC[l] = m.KC;
                 for (unsigned z = 0; z < m.grid.z; z++)
                     for (unsigned y = 0; y < m.grid.y; y++)
@@ -342,26 +349,26 @@ private:
         HIPCHK(hipStreamSynchronize(s_));
     }
 
-    // Autotuner: the march work-list density (workgroups aimed at per level
-    // and grid) decides the plane chunk of every march launch.  Each
+    // Autotuner over the launch shapes, by coordinate descent: the march
+    // work-list density (workgroups aimed at per level and grid: 64, 256,
+    // 1024), then the fewest planes per march chunk (2, 4, 8), then the planes
+    // per level-0 row tile of 3D members (the engine's choice, 8, 16).  Each
     // candidate is timed on a few V-cycles of the real members (zero right-
     // hand sides: the kernels run all their arithmetic regardless) and the
-    // fastest is kept for the batch.
+    // fastest is kept for the batch.  Every candidate computes the same
+    // arithmetic; only the tiling changes.
     void autotune() {
         tuned_ = true;
-        const long cand[] = {64, 256, 1024};
         const int G = size();
-        // a zero rhs is enough: time only the V-cycle work-lists
+        bool any_rows = false;
+        for (int g = 0; g < G; g++) any_rows = any_rows || (D == 3 && hd_[g].cpl);
         std::vector<BGrid<S>> save = hd_;
-        kb_ctrl_init<S><<<(G + 63) / 64, 64, 0, s_>>>(dd_, G, 1 << 30, 0.0, o_.tol_mode, 0, o_.project_mean);
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        float best = 1e30f;
-        long best_t = target_;
-        for (long t : cand) {
+        auto time_it = [&]() {
             free_lists();
-            make_lists(t);
+            make_lists(target_);
             kb_ctrl_init<S><<<(G + 63) / 64, 64, 0, s_>>>(dd_, G, 1 << 30, 0.0, o_.tol_mode, 0, o_.project_mean);
             vcycle();   // warm-up
             HIPCHK(hipEventRecord(e0, s_));
@@ -370,11 +377,34 @@ private:
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; best_t = t; }
+            return ms;
+        };
+        float best = 1e30f;
+        long bt = target_;
+        for (long t : {64L, 256L, 1024L}) {
+            target_ = t;
+            const float ms = time_it();
+            if (ms < best) { best = ms; bt = t; }
+        }
+        target_ = bt;
+        int bk = kc_min_;
+        for (int k : {4, 8}) {
+            kc_min_ = k;
+            const float ms = time_it();
+            if (ms < best) { best = ms; bk = k; }
+        }
+        kc_min_ = bk;
+        if (any_rows) {
+            int br = rows_kc_;
+            for (int r : {8, 16}) {
+                rows_kc_ = r;
+                const float ms = time_it();
+                if (ms < best) { best = ms; br = r; }
+            }
+            rows_kc_ = br;
         }
         HIPCHK(hipEventDestroy(e0));
         HIPCHK(hipEventDestroy(e1));
-        target_ = best_t;
         free_lists();
         hd_ = save;
         make_lists(target_);
@@ -432,6 +462,8 @@ private:
     std::vector<std::unique_ptr<E>> eng_;
     bool dirty_ = true, tuned_ = false, done_ = true, any_warm_ = false;
     long target_ = 256;
+    int kc_min_ = 2;      // fewest planes per march chunk
+    int rows_kc_ = 0;     // planes per level-0 row tile (0: the engine's choice)
     int max_iter_ = 0, queued_ = 0;
     std::vector<BGrid<S>> hd_;
     BGrid<S>* dd_ = nullptr;

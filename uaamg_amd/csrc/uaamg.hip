@@ -831,6 +831,16 @@ extern "C" int uaamg_batch_clear(uaamg_batch* b) {
 
 extern "C" int uaamg_batch_size(const uaamg_batch* b) { return b ? (int)b->members.size() : 0; }
 
+extern "C" int uaamg_batch_tuning(const uaamg_batch* b, int dim, int* march_target, int* march_kc_min, int* rows_kc) {
+    if (!b || (dim != 2 && dim != 3)) return UAAMG_ERR_ARG;
+    const auto& p = b->part[dim == 3 ? 1 : 0];
+    if (!p) return UAAMG_ERR_ARG;
+    if (march_target) *march_target = p->tuned_target();
+    if (march_kc_min) *march_kc_min = p->tuned_kc_min();
+    if (rows_kc) *rows_kc = p->tuned_rows_kc();
+    return UAAMG_OK;
+}
+
 extern "C" size_t uaamg_batch_bytes(const uaamg_batch* b) {
     if (!b) return 0;
     size_t s = 0;
