@@ -613,9 +613,20 @@ public:
             Level& F = lv_[l];
             Level& C = lv_[l + 1];
             const RowGeo& rg = rgl_[l];
+            FinUpd fin{};
+            if (fin_fold_ && l == 1) { fin = FinUpd{partials_, (int)rows_blocks_, ctrl_}; fin_fold_ = false; }
             k_rows_downc<2, ROWS_NW, ROWS_RW, S, C4><<<8u * (unsigned)rg.per_xcd, ROWS_NW * 64, 0, stream_>>>(
-                F.b, F.coef, F.g, rg, C.b, C.g, skip);
+                F.b, F.coef, F.g, rg, C.b, C.g, skip, fin);
         }
+    }
+    // the convergence test of the split update runs in the level-1 row
+    // kernel instead of its own one-workgroup launch (k_fin_update, 5.2 us
+    // per iteration at 256^3) when level 1 is on the row kernels; set by
+    // rows_down0, taken by rows_downc
+    bool fin_fold_ = false;
+    bool fin_foldable() const {
+        const int lt = tail_first_ ? tail_first_ : nlevels() - 1;
+        return nlevels() > 2 && lt > 1 && rcpl_[1] != 0;
     }
     void rows_upc(int l, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
@@ -638,7 +649,8 @@ public:
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
                         rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const double*)partials_m_,
                         (int)pmv_blocks_, ctrl_);
-                k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
+                fin_fold_ = fin_foldable();
+                if (!fin_fold_) k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
                 return;
             }
             tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? k_rows_down0<4, ROWS_NW, ROWS_RW, S> : k_rows_down0<2, ROWS_NW, ROWS_RW, S>,

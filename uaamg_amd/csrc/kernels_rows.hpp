@@ -201,6 +201,14 @@ struct RowThread {
 // as the z-sums finish stored it); workgroup 0 records p.Ap, alpha, the x
 // update's alpha, a breakdown, and rsold = (r, z) for the next beta, in
 // scalars that no other workgroup of this launch reads.
+// the convergence test (k_fin_update) folded into the next kernel: the r.r
+// partials it sums and the scalars (ctrl == nullptr: not folded)
+struct FinUpd {
+    const double* partials = nullptr;
+    int nb = 0;
+    Ctrl* ctrl = nullptr;
+};
+
 struct RUpd {
     const float* Ap = nullptr;
     float* rnew = nullptr;
@@ -821,9 +829,42 @@ __device__ __forceinline__ void rec_colour(RecQ<CPL, RW>& Q, RecLDS<CPL, NW, RW>
 // b (level l) -> coarse b: forward sweep and restricted residual
 template <int CPL, int NW, int RW, typename SV, typename C4>
 __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4* __restrict__ rec, const Geo& g,
-                                             const RowGeo& rg, SV* __restrict__ bc, const Geo& cg, const int* skip, int tile) {
+                                             const RowGeo& rg, SV* __restrict__ bc, const Geo& cg, const int* skip, int tile,
+                                             const FinUpd& fin = FinUpd{}) {
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
-    if (skip && *skip) return;
+    if (fin.ctrl) {
+        // the convergence test of the level-0 down kernel's r.r partials
+        // (k_fin_update) in every workgroup: each sums them in the same order
+        // and stops on the same r.r; workgroup 0 records it.  Only workgroup
+        // 0 reads the iteration count (it increments it): a max_iter stop is
+        // seen by the following kernels, and any work this launch does after
+        // it is unused.
+        // (done is read once per workgroup: workgroup 0 of this launch may
+        // set it while others start, and the whole workgroup must agree)
+        __shared__ int s_done;
+        __shared__ double s_rs;
+        if (threadIdx.x == 0) s_done = fin.ctrl->done;
+        __syncthreads();
+        if (s_done) return;
+        double tot[1];
+        reduce_partials_at<1>(fin.partials, fin.nb, 0, fin.nb, tot);
+        if (threadIdx.x == 0) s_rs = tot[0];
+        __syncthreads();
+        const double rs = s_rs;
+        Ctrl* c = fin.ctrl;
+        const bool conv = (c->tol_mode == 1) ? (rs < c->tolsq) : (sqrt(rs) < c->tol);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c->rsnew = rs;
+            if (c->hist) c->hist[c->iter] = rs;
+            c->iter += 1;
+            if (conv) { c->done = 1; c->status = ST_CONVERGED; }
+            else if (!isfinite(rs)) { c->done = 1; c->status = ST_NONFINITE; }
+            else if (c->iter >= c->max_iter) { c->done = 1; c->status = ST_MAXITER; }
+        }
+        if (conv || !isfinite(rs)) return;
+    } else if (skip && *skip) {
+        return;
+    }
     __shared__ RecLDS<CPL, NW, RW> L;
     if (tile >= rg.ntiles) return;
     const int ty = tile % rg.nty, kc = tile / rg.nty;
@@ -934,8 +975,9 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
 
 template <int CPL, int NW, int RW, typename SV, typename C4>
 __global__ __launch_bounds__(NW * 64) void k_rows_downc(const SV* __restrict__ b, const C4* __restrict__ rec, Geo g,
-                                                        RowGeo rg, SV* __restrict__ bc, Geo cg, const int* skip) {
-    d_rows_downc<CPL, NW, RW, SV, C4>(b, rec, g, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x));
+                                                        RowGeo rg, SV* __restrict__ bc, Geo cg, const int* skip,
+                                                        FinUpd fin) {
+    d_rows_downc<CPL, NW, RW, SV, C4>(b, rec, g, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), fin);
 }
 
 // b (level l), coarse x -> x (level l): prolongation and reverse sweep
