@@ -52,6 +52,8 @@ struct Ctrl {
     double sum_r, sum_z, sum_rz;
     double rz_cross;       // (r_new, z_old - mean_old) for the flexible beta
     double xalpha;         // alpha of this iteration's x update; 0 when the operator finish found the solve stopped
+    double xa[2];          // the same for the split update's x pass, per p buffer: the x pass of one
+                           // iteration may still run while the next iteration forms its alpha
     int iter, max_iter, done, status, first, tol_mode, project, flexible;
     double* hist;          // optional per-iteration r.r history (nullptr: off)
 };

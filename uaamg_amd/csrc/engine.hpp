@@ -648,7 +648,7 @@ public:
                                                               : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
                         rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const double*)partials_m_,
-                        (int)pmv_blocks_, ctrl_);
+                        (int)pmv_blocks_, ctrl_, xslot_);
                 fin_fold_ = fin_foldable();
                 if (!fin_fold_) k_fin_update<false><<<1, FIN_THREADS, 0, stream_>>>(partials_, (int)rows_blocks_, ctrl_);
                 return;
@@ -698,13 +698,21 @@ public:
                 Level& F = lv_[l];
                 FusedArgs<D, false, SC, C4> A{F.g, nullptr, F.coef, Consts<SC>{}};
                 MLaunch m = mlaunch(F.g, RXC, RYC);
-                k_down_m<D, false, RXC, RYC, PPTC, S, C4, S, C4><<<m.grid, 256, 0, stream_>>>(
-                    F.b, A, C.b, C.coef, C.g, m.KC, skip);
+                auto kern = k_down_m<D, false, RXC, RYC, PPTC, S, C4, S, C4>;
+                if (fr_old_ && xfork_level() == l && !timer_.on && !capturing_) {
+                    // the fork event rides on this dispatch's completion: no
+                    // marker packet between it and the next kernel on stream_
+                    hipExtLaunchKernelGGL(kern, m.grid, dim3(256), 0, stream_, nullptr, xfork_, 0,
+                                          F.b, A, C.b, C.coef, C.g, m.KC, skip);
+                    fork_recorded_ = true;
+                } else {
+                    kern<<<m.grid, 256, 0, stream_>>>(F.b, A, C.b, C.coef, C.g, m.KC, skip);
+                }
             }
             HIPCHK(hipGetLastError());
             if (fr_old_ && l > 0 && xfork_level() == l) fork_x();
         }
-        if (fr_old_ && xpend_ == false && xfork_level() >= lt) fork_x();   // before the tail / coarsest solve
+        if (fr_old_ && !xforked_ && xfork_level() >= lt) fork_x();   // before the tail / coarsest solve
         if (tail_first_) {
             TScope ts(timer_, UAAMG_K_COARSE);
             tail_launch(skip);
@@ -932,8 +940,10 @@ public:
                 const int f0 = forms_used_;
                 forms_used_ = 0;
                 HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                capturing_ = true;
                 for (int c = 0; c < chunk; c++) iteration(x, skip);
                 join_x();   // every fork joins the capturing stream before the capture ends
+                capturing_ = false;
                 HIPCHK(hipStreamEndCapture(stream_, &graph_[k]));
                 HIPCHK(hipGraphInstantiate(&gexec_[k], graph_[k], nullptr, nullptr, 0));
                 graph_valid_ = true;
@@ -954,7 +964,15 @@ public:
         const unsigned fb = flat_blocks(2048);
         const T* pv = p_;        // the p of this iteration's operator and update
         unsigned mb = fb;        // workgroups of the operator launch (its partials)
-        join_x();                // the previous iteration's x update has finished
+        // With the split update, the x pass of iteration k reads p_k (one of
+        // the two p buffers) and its alpha slot Ctrl::xa[buffer]; the operator
+        // pass of iteration k+2 is the first to overwrite either.  So the main
+        // stream joins the x stream only before the operator passes that write
+        // buffer 0: that join covers the x passes of the two iterations before
+        // it, and a join costs a barrier packet on the main queue (about 6 us
+        // at 256^3).  Other forms join every iteration.
+        xforked_ = false;
+        if (!fr_ || pcur_ == 1) join_x();
         if (pmv_on_) {
             const T* pold = pbuf(pcur_);
             T* pnew = pbuf(pcur_ ^ 1);
@@ -985,6 +1003,7 @@ public:
             fr_old_ = rbuf(pcur_);
             xfork_x_ = x;
             xfork_p_ = pv;
+            xslot_ = pcur_ ^ 1;   // the buffer pv is in
             T* rnew = rbuf(pcur_ ^ 1);
             pcur_ ^= 1;
             vcycle(rnew, z_, skip);   // forks the x update after the level-0 down kernel
@@ -1167,8 +1186,13 @@ private:
     bool xpend_ = false;
     T* xfork_x_ = nullptr;
     const T* xfork_p_ = nullptr;
+    bool fork_recorded_ = false;   // xfork_ was recorded by the preceding dispatch itself
+    bool xforked_ = false;         // this iteration's x pass was forked
+    int xslot_ = 0;                // Ctrl::xa slot (p buffer) of this iteration's x pass
+    bool capturing_ = false;       // a hipGraph capture of stream_ is open
     void fork_x() {
-        HIPCHK(hipEventRecord(xfork_, stream_));
+        if (!fork_recorded_) HIPCHK(hipEventRecord(xfork_, stream_));
+        fork_recorded_ = false;
         HIPCHK(hipStreamWaitEvent(xs_, xfork_, 0));
         const unsigned xthreads = (unsigned)std::min(1024, std::max(64, o_.xthreads / 64 * 64));
         const long xneed = (N4_ / 4 + xthreads - 1) / xthreads;
@@ -1180,9 +1204,10 @@ private:
         const bool xnt = !xnb && (o_.forms & UAAMG_FORM_NT_X);
         forms_used_ |= (xnb ? UAAMG_FORM_XMASK : 0) | (xnt ? UAAMG_FORM_NT_X : 0);
         tlaunch(timer_, UAAMG_K_UPDATE, xnb ? k_xupdate<T, true> : (xnt ? k_xupdate<T, false, true> : k_xupdate<T, false>), dim3(xb),
-                dim3(xthreads), xs_, xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const Ctrl*)ctrl_);
+                dim3(xthreads), xs_, xfork_x_, xfork_p_, (const uint8_t*)nbr0_, N4_, (const double*)&ctrl_->xa[xslot_]);
         HIPCHK(hipEventRecord(xjoin_, xs_));
         xpend_ = true;
+        xforked_ = true;
     }
     void join_x() {
         if (!xpend_) return;

@@ -474,15 +474,15 @@ __global__ void k_fin_zsums(const double* partials, int nb, Ctrl* ctrl) {
     d_fin_zsums(partials, nb, 0, nb, ctrl);
 }
 
-// x += alpha p on fluid cells, alpha = ctrl->xalpha (the x half of k_update,
+// x += alpha p on fluid cells, alpha = *xalpha (a Ctrl::xa slot; the x half of k_update,
 // launched on its own; it does not read the done flag, which the r half may
 // set while it runs).  With NB = false the mask is not read: for a p that is
 // exactly 0 on every non-fluid cell and ghost (the operator pass's p), the
 // update leaves those cells of x unchanged by itself.
 template <typename T, bool NB = true, bool NT = false>
 __global__ void k_xupdate(T* __restrict__ x, const T* __restrict__ p, const uint8_t* __restrict__ nbr, long N4,
-                          const Ctrl* ctrl) {
-    const double alpha = ctrl->xalpha;
+                          const double* xalpha) {
+    const double alpha = *xalpha;
     if (alpha == 0.0) return;
     const int fv_b = blockIdx.x, fv_n = gridDim.x;
     FLAT_LOOP(N4) {

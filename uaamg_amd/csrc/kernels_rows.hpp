@@ -216,6 +216,7 @@ struct RUpd {
     const double* pap = nullptr;     // p.Ap partials of the operator pass
     int npap = 0;
     Ctrl* ctrl = nullptr;
+    int xslot = 0;                   // ctrl->xa slot of this iteration's x pass (its p buffer)
 };
 
 template <int CPL, int NW, int RW, typename VC, bool FR = false, bool NT = false>
@@ -225,7 +226,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
     static_assert(CPL % 2 == 0 && RW % 2 == 0, "pairs of cells and rows");
     if (skip && *skip) {
         // a stopped solve: no x update for this iteration
-        if constexpr (FR) { if (blockIdx.x == 0 && threadIdx.x == 0) u.ctrl->xalpha = 0.0; }
+        if constexpr (FR) { if (blockIdx.x == 0 && threadIdx.x == 0) u.ctrl->xa[u.xslot] = 0.0; }
         return;
     }
     __shared__ RowLDS<CPL, NW, RW> L;
@@ -244,7 +245,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
             if (!isfinite(pAp)) { c->done = 1; c->status = ST_NONFINITE; c->iter += 1; c->alpha = 0.0; }
             else if (pAp < 1e-15) { c->done = 1; c->status = ST_BREAKDOWN; c->iter += 1; c->alpha = 0.0; }
             else c->alpha = c->rz / pAp;
-            c->xalpha = c->alpha;
+            c->xa[u.xslot] = c->alpha;
             // the operator pass read rsold and the first flag; nothing in
             // this launch does
             c->rsold = c->rz;
@@ -451,9 +452,9 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict
                                                          Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
                                                          const int* skip, const float* __restrict__ Ap,
                                                          float* __restrict__ rnew, double* partials,
-                                                         const double* pap, int npap, Ctrl* ctrl) {
+                                                         const double* pap, int npap, Ctrl* ctrl, int xslot) {
     RUpd u;
-    u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.pap = pap; u.npap = npap; u.ctrl = ctrl;
+    u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.pap = pap; u.npap = npap; u.ctrl = ctrl; u.xslot = xslot;
     d_rows_down0<CPL, NW, RW, VC, true, NT>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
 }
 
