@@ -594,6 +594,7 @@ def main():
     ap.add_argument("--keys-only", action="store_true",
                     help="only the 1024x512 2D solve under each PCG key (pcg_uaamg, pcg_gmg, pcg_amg, cg)")
     ap.add_argument("--no-project", action="store_true", help="skip the projection measurement")
+    ap.add_argument("--no-keep-fluid", action="store_true", help="skip the keep-fluid hierarchy measurement")
     ap.add_argument("--project-only", action="store_true", help="only the projection measurement")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this benchmark and exit")
@@ -759,6 +760,48 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_keep_fluid:
+        # the same workload with the keep-fluid hierarchy (a coarse cell is solid
+        # only when all its children are; no counterpart in the reference):
+        # reported beside the reference-compatible `value`, never as it
+        p_ref = p_d.clone()
+        ck = U.Context(precision=prec, tol_mode=U.TOL_REL, max_iter=1000, tol=1e-6, check_every=args.check_every,
+                       hierarchy_mode=U.HIERARCHY_KEEP_FLUID)
+        ck.n = pb.n; ck.dim = pb.dim; ck.N = pb.N
+
+        def step_kf():
+            rc = ck.lib.L.uaamg_set_grid_device(ck.h, ctypes.byref(grid))
+            if rc != 0:
+                raise RuntimeError(f"set_grid_device: {U.status_name(rc)}")
+            p_d.zero_()
+            return ck.solve_device(rhs_d.data_ptr(), p_d.data_ptr(), 0, 1000, 1e-6)
+
+        for _ in range(max(1, args.warmup)):
+            step_kf()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kf_iters = []
+        for _ in range(args.steps):
+            stk = step_kf()
+            kf_iters.append(stk.iters)
+            if stk.status != 0:
+                raise RuntimeError(f"keep-fluid solve did not converge: {U.status_name(stk.status)}")
+        torch.cuda.synchronize()
+        t_kf = (time.perf_counter() - t0) / args.steps
+        ck.close()
+        fl = torch.from_numpy(pb.fluid_mask()).to(dev)
+        a, b = p_d[fl], p_ref[fl]
+        a, b = a - a.mean(), b - b.mean()
+        out["keep_fluid_hierarchy"] = {
+            "hierarchy_mode": "keep_fluid (a coarse cell is solid only when all its children are; "
+                              "not the reference's algorithm, so not `value`)",
+            "solves_per_s": round(1.0 / t_kf, 3),
+            "ms_per_step": round(1e3 * t_kf, 3),
+            "iters": int(np.median(kf_iters)),
+            "iters_reference_hierarchy": k_med,
+            "max_rel_diff_vs_reference_hierarchy": float((a - b).abs().max() / b.abs().max()),
+        }
+        log(f"[bench] keep-fluid hierarchy: {out['keep_fluid_hierarchy']}")
     if not args.no_project or not args.no_batch:
         del rhs_d, solid_d, p_d
         ctx.close()

@@ -91,9 +91,20 @@ typedef struct {
                                   factory has no such keys, factory_3d.cpp:21-23: a 3D grid is
                                   UAAMG_ERR_ARG) and run V(2,2) whatever nu_pre / nu_post say
                                   (gmg_preconditioner.cpp:163-174); changing it drops the grid */
+    int hierarchy_mode;        /* UAAMG_HIERARCHY_REFERENCE (default): a coarse cell is solid when
+                                  at least 2 of its 4 (2D) or 4 of its 8 (3D) children are
+                                  (restrictSolid, uaamg_preconditioner.cpp:43-53, _3d.cpp:34-45;
+                                  gmg/amg_preconditioner.cpp:29-39 / :141-155), which drops fluid
+                                  from the coarse grids and leaves one-sided couplings into
+                                  coarse-solid cells.  UAAMG_HIERARCHY_KEEP_FLUID: solid only when
+                                  every child is; the Galerkin coarse operators are then exactly
+                                  the aggregation operators R^T A P.  No counterpart in the
+                                  reference: results then differ from the reference's (fewer
+                                  iterations where solids are present).  Changing it drops the grid */
 } uaamg_opts;
 
 enum { UAAMG_PRECOND_UAAMG = 0, UAAMG_PRECOND_NONE = 1, UAAMG_PRECOND_GMG = 2, UAAMG_PRECOND_AMG = 3 };
+enum { UAAMG_HIERARCHY_REFERENCE = 0, UAAMG_HIERARCHY_KEEP_FLUID = 1 };
 
 /* kernel forms (uaamg_opts.forms, uaamg_stats.forms_used) */
 enum {

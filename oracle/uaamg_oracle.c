@@ -29,6 +29,8 @@
  *     coarsened as the sum over children (R^T A P of a diagonal term);
  *     mean projection off.
  *   - nu_pre/nu_post sweeps, coarsest side and coarsest sweep count.
+ *   - hierarchy_mode 1: a coarse cell is solid only when every child is
+ *     (restrict_solid; the reference's threshold is mode 0).
  *   - relative tolerance mode (the GPU solve_optimized rule).
  */
 #include "uaamg_oracle.h"
@@ -60,6 +62,7 @@ void orc_default_opts(orc_opts* o) {
     o->tol_mode = 0;
     o->project_mean = -1;
     o->precond = ORC_PRECOND_UAAMG;
+    o->hierarchy_mode = 0;
 }
 
 static inline size_t IX(const Level* L, int i, int j, int k) {
@@ -114,8 +117,8 @@ static int build_hier(Hier* H, const orc_grid* g, const orc_opts* o) {
     return H->nl > 0 ? 0 : -3;
 }
 
-static void restrict_solid(int dim, const Level* F, Level* C) {
-    int thr = (dim == 3) ? 4 : 2;
+static void restrict_solid(int dim, const Level* F, Level* C, int mode) {
+    int thr = mode ? (dim == 3 ? 8 : 4) : (dim == 3) ? 4 : 2;
     for (int ic = 1; ic <= C->nx; ic++)
         for (int jc = 1; jc <= C->ny; jc++)
             for (int kc = KLO(dim, C); kc <= KHI(dim, C); kc++) {
@@ -320,7 +323,7 @@ static void vcycle(Hier* H, int level) {
 static void setup_levels(Hier* H, const orc_grid* g) {
     Level* L0 = &H->L[0];
     for (size_t c = 0; c < L0->N; c++) L0->solid[c] = g->solid[c] ? 1 : 0;
-    for (int l = 1; l < H->nl; l++) restrict_solid(H->dim, &H->L[l - 1], &H->L[l]);
+    for (int l = 1; l < H->nl; l++) restrict_solid(H->dim, &H->L[l - 1], &H->L[l], H->o.hierarchy_mode);
     setup_fine(H, L0);
     for (int l = 1; l < H->nl; l++) galerkin(H->dim, &H->L[l - 1], &H->L[l]);
 }
@@ -424,7 +427,7 @@ typedef struct {
     size_t N;
 } GLevel;
 
-typedef struct { int nl, amg, csweeps; GLevel L[MAXL]; } GHier;
+typedef struct { int nl, amg, csweeps, keep_fluid; GLevel L[MAXL]; } GHier;
 
 #define GX2(L, i, j) ((size_t)(i) + (size_t)(j) * (size_t)((L)->nx + 2))
 
@@ -436,7 +439,7 @@ static void gmg_free(GHier* H) {
 static int gmg_build(GHier* H, const orc_grid* g, const orc_opts* o, int amg) {
     int nx = g->nx, ny = g->ny;
     double dx = g->dx, dy = g->dy;
-    H->nl = 0; H->amg = amg; H->csweeps = o->coarsest_sweeps;
+    H->nl = 0; H->amg = amg; H->csweeps = o->coarsest_sweeps; H->keep_fluid = o->hierarchy_mode;
     while (nx >= 2 && ny >= 2 && H->nl < MAXL) {
         GLevel* L = &H->L[H->nl++];
         L->nx = nx; L->ny = ny; L->dx = dx; L->dy = dy;
@@ -546,14 +549,14 @@ static void gmg_vcycle(GHier* H, int l) {
 static void gmg_apply(GHier* H, const orc_grid* g, const double* r, double* z) {
     GLevel* L0 = &H->L[0];
     for (size_t c = 0; c < L0->N; c++) L0->solid[c] = g->solid[c] ? 1 : 0;
-    for (int l = 1; l < H->nl; l++) {   /* at least half of the 2x2 children solid */
+    for (int l = 1; l < H->nl; l++) {   /* at least half of the 2x2 children solid (all with hierarchy_mode 1) */
         GLevel *F = &H->L[l - 1], *C = &H->L[l];
         for (int ic = 1; ic <= C->nx; ic++)
             for (int jc = 1; jc <= C->ny; jc++) {
                 int sc = 0;
                 for (int di = 0; di < 2; di++)
                     for (int dj = 0; dj < 2; dj++) sc += F->solid[GX2(F, 2 * ic - 1 + di, 2 * jc - 1 + dj)] ? 1 : 0;
-                C->solid[GX2(C, ic, jc)] = sc >= 2;
+                C->solid[GX2(C, ic, jc)] = sc >= (H->keep_fluid ? 4 : 2);
             }
     }
     for (int i = 1; i <= L0->nx; i++)

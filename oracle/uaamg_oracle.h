@@ -46,6 +46,9 @@ typedef struct {
                                     (key "cg": IdentityPreconditioner, z = r; factory.cpp:19),
                                     ORC_PRECOND_GMG (keys "pcg", "pcg_gmg": GMGPreconditioner, 2D) or
                                     ORC_PRECOND_AMG (key "pcg_amg": AMGPreconditioner, 2D) */
+    int hierarchy_mode;          /* 0: the reference's coarse-solid rule (at least 2 of 4 / 4 of 8
+                                    children solid, restrictSolid); 1: coarse cell solid only when
+                                    every child is (no fluid dropped from the coarse grids) */
 } orc_opts;
 
 #define ORC_PRECOND_UAAMG    0

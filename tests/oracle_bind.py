@@ -25,7 +25,7 @@ class OrcGrid(C.Structure):
 class OrcOpts(C.Structure):
     _fields_ = [("nu_pre", C.c_int), ("nu_post", C.c_int), ("coarsest_side", C.c_int),
                 ("coarsest_sweeps", C.c_int), ("tol_mode", C.c_int), ("project_mean", C.c_int),
-                ("precond", C.c_int)]
+                ("precond", C.c_int), ("hierarchy_mode", C.c_int)]
 
 
 PRECOND_UAAMG, PRECOND_IDENTITY, PRECOND_GMG, PRECOND_AMG = 0, 1, 2, 3

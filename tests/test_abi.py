@@ -54,7 +54,7 @@ int main(void) {
   F(uaamg_opts, project_mean) F(uaamg_opts, fused) F(uaamg_opts, check_every) F(uaamg_opts, use_graph)
   F(uaamg_opts, timing) F(uaamg_opts, flexible) F(uaamg_opts, forms) F(uaamg_opts, xfork_level)
   F(uaamg_opts, xblocks) F(uaamg_opts, xthreads) F(uaamg_opts, rows_kc) F(uaamg_opts, march_kc_min)
-  F(uaamg_opts, march_target) F(uaamg_opts, trace) F(uaamg_opts, preconditioner)
+  F(uaamg_opts, march_target) F(uaamg_opts, trace) F(uaamg_opts, preconditioner) F(uaamg_opts, hierarchy_mode)
   printf("UAAMG_FORM_DEFAULT %d\n", (int)UAAMG_FORM_DEFAULT);
   F(uaamg_stats, iters) F(uaamg_stats, status) F(uaamg_stats, nlevels) F(uaamg_stats, rnorm)
   F(uaamg_stats, r0norm) F(uaamg_stats, ms_total) F(uaamg_stats, ms_setup) F(uaamg_stats, ms_kernel)
@@ -125,7 +125,7 @@ def test_bad_arguments_rejected():
 @pytest.mark.parametrize("kw", [{"precision": U.MIXED_FP16}, {"precision": 6}, {"forms": 1 << 12},
                                 {"xthreads": 32}, {"xblocks": 0}, {"march_target": 0}, {"rows_kc": -4},
                                 {"tol_mode": 2}, {"nu_pre": -1}, {"preconditioner": 4},
-                                {"preconditioner": -1}])
+                                {"preconditioner": -1}, {"hierarchy_mode": 2}])
 def test_invalid_options_are_argument_errors(kw):
     """Options are checked before the device: an unbuilt precision (FP16
     levels) or an out-of-range form or launch shape is UAAMG_ERR_ARG, not a

@@ -566,3 +566,118 @@ def test_gmg_and_amg_agree_to_rounding():
         a, b = pg[m] - pg[m].mean(), pa[m] - pa[m].mean()
         assert rg.status == 0 and ra.status == 0 and abs(rg.iters - ra.iters) <= 1, name
         assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max(), name
+
+
+# ---------------------------------------------------------------- hierarchy_mode 1 (keep fluid)
+
+def _aggregation_P(pb, c):
+    """P: fluid fine cell -> its coarse parent (constant injection over the
+    2^d children), columns over every coarse interior cell."""
+    A, idx = dense_level0(pb)
+    cshape = tuple(v + 2 for v in c)
+    pos = {cc: q for q, cc in enumerate(idx)}
+    cidx = []
+    P = np.zeros((len(idx), int(np.prod(c))))
+    grid = np.stack(np.meshgrid(*[np.arange(1, v + 1) for v in c[::-1]], indexing="ij"), -1).reshape(-1, pb.dim)
+    for q, cc in enumerate(grid):
+        ic = cc[::-1]
+        cidx.append(int(sum(ic[a] * int(np.prod(cshape[:a])) for a in range(pb.dim))))
+        for off in np.ndindex(*([2] * pb.dim)):
+            f = [2 * ic[a] - 1 + off[a] for a in range(pb.dim)]
+            fid = int(sum(f[a] * int(np.prod(pb.shape[:a])) for a in range(pb.dim)))
+            if fid in pos:
+                P[pos[fid], q] = 1.0
+    return A, P, np.array(cidx), cshape
+
+
+@pytest.mark.parametrize("n,L,blocks", [((8, 6), (1.0, 0.75), [(2, 5, 1, 4)]), ((12, 10), (1.0, 1.0), [(3, 8, 2, 5), (9, 11, 6, 9)]),
+                                        ((8, 6, 6), (1.0, 0.75, 0.5), [(2, 5, 1, 4, 2, 5)]),
+                                        ((10, 8, 6), (1.0, 1.0, 0.8), [(3, 7, 2, 5, 1, 4)])])
+def test_keep_fluid_galerkin_is_RtAP_with_solids(n, L, blocks):
+    """hierarchy_mode 1: a coarse cell is solid only when all its children
+    are, so every coarse coupling is the aggregation Galerkin product
+    P^T A P, solids included (under the reference's threshold a coarse
+    fluid cell keeps couplings into half-solid coarse cells, which no P^T A P
+    has).  Blocks that cover some children of a coarse cell test it."""
+    pb = ob.Problem(n, L)
+    s = pb.view(pb.solid)
+    for b in blocks:
+        if pb.dim == 2:
+            s[b[2]:b[3], b[0]:b[1]] = 1
+        else:
+            s[b[4]:b[5], b[2]:b[3], b[0]:b[1]] = 1
+    A, P, cidx, cshape = _aggregation_P(pb, ob.hierarchy_dims(pb)[1])
+    Ac = P.T @ A @ P
+    lv = ob.level_data(pb, 1, hierarchy_mode=1)
+    fluid_c = P.sum(axis=0) > 0
+    assert np.array_equal(lv["solid"][cidx] == 0, fluid_c)          # solid iff no fluid child
+    assert np.allclose(lv["diag"][cidx][fluid_c], np.diag(Ac)[fluid_c], rtol=1e-13)
+    cstr = [1, cshape[0]] + ([cshape[0] * cshape[1]] if pb.dim == 3 else [])
+    where = {v: q for q, v in enumerate(cidx)}
+    checked = 0
+    for q, cc in enumerate(cidx):
+        if not fluid_c[q]:
+            continue
+        for ax, ch in enumerate(["cx", "cy", "cz"][:pb.dim]):
+            nb = cc + cstr[ax]
+            if nb in where:
+                assert np.isclose(-Ac[q, where[nb]], lv[ch][cc], rtol=1e-13, atol=1e-9), (q, ch)
+                checked += 1
+    assert checked > 0
+    # the reference's threshold does not give P^T A P here
+    ref = ob.level_data(pb, 1)
+    assert not np.array_equal(ref["solid"][cidx] == 0, fluid_c)
+
+
+def test_keep_fluid_threshold():
+    """Coarse solid only when every child is: 3 of 4 (2D) / 7 of 8 (3D)
+    solid children leave the coarse cell fluid."""
+    pb = ob.Problem((4, 4))
+    s = np.zeros(pb.shape[::-1], np.uint8)
+    s[1, 1] = s[1, 2] = s[2, 1] = 1              # 3 of 4 -> fluid
+    s[1, 3] = s[1, 4] = s[2, 3] = s[2, 4] = 1    # 4 of 4 -> solid
+    pb.solid = s.reshape(-1)
+    cs = ob.level_data(pb, 1, coarsest_side=2, hierarchy_mode=1)["solid"].reshape(4, 4)
+    assert cs[1, 1] == 0 and cs[1, 2] == 1
+    pb3 = ob.Problem((4, 4, 4))
+    s3 = np.zeros(pb3.shape[::-1], np.uint8)
+    s3[1:3, 1:3, 1:3] = 1; s3[2, 2, 2] = 0        # 7 of 8 -> fluid
+    s3[1:3, 1:3, 3:5] = 1                          # 8 of 8 -> solid
+    pb3.solid = s3.reshape(-1)
+    cs3 = ob.level_data(pb3, 1, coarsest_side=2, hierarchy_mode=1)["solid"].reshape(4, 4, 4)
+    assert cs3[1, 1, 1] == 0 and cs3[1, 1, 2] == 1
+
+
+@pytest.mark.parametrize("name,pb", [c for c in cases.parity_cases() if "porous" not in c[0]],
+                         ids=[c[0] for c in cases.parity_cases() if "porous" not in c[0]])
+def test_keep_fluid_vcycle_symmetric_and_not_slower(name, pb):
+    """hierarchy_mode 1 keeps the V-cycle symmetric (<M r1, r2> = <r1, M r2>)
+    and never needs more PCG iterations than the reference's hierarchy on
+    the parity geometries (fewer where solids cut coarse cells)."""
+    rng = np.random.default_rng(8)
+    m = pb.fluid_interior_mask()
+    r1, r2 = cases.random_rhs(pb, rng), cases.random_rhs(pb, rng)
+    r1[~m] = 0; r2[~m] = 0
+    a = np.dot(ob.uaamg_apply(pb, r1, hierarchy_mode=1)[m], r2[m])
+    b = np.dot(r1[m], ob.uaamg_apply(pb, r2, hierarchy_mode=1)[m])
+    assert abs(a - b) <= 1e-11 * max(abs(a), abs(b))
+    rhs = 1e3 * cases.random_rhs(pb, np.random.default_rng(3))
+    _, r0 = ob.pcg(pb, rhs, 2000, 1e-6, tol_mode=1)
+    p1, r1_ = ob.pcg(pb, rhs, 2000, 1e-6, tol_mode=1, hierarchy_mode=1)
+    assert r0.status == 0 and r1_.status == 0
+    assert r1_.iters <= r0.iters, (r1_.iters, r0.iters)
+    # and solves the same system: the two pressures agree to the tolerance's accuracy
+    p0, _ = ob.pcg(pb, rhs, 2000, 1e-10, tol_mode=1)
+    p1, _ = ob.pcg(pb, rhs, 2000, 1e-10, tol_mode=1, hierarchy_mode=1)
+    x, y = p0[m] - p0[m].mean(), p1[m] - p1[m].mean()
+    assert np.abs(x - y).max() <= 1e-7 * np.abs(x).max()
+
+
+def test_keep_fluid_plume_iterations():
+    """The 64^3 plume (8 spheres and a floor slab) at relres 1e-6: the
+    reference's hierarchy needs 35 iterations, hierarchy_mode 1 needs 12."""
+    wp = W.plume3d(seed=0, n=64)
+    pb = ob.Problem(wp.n, wp.L, wp.solid, wp.wall)
+    _, r0 = ob.pcg(pb, wp.rhs, 500, 1e-6, tol_mode=1)
+    _, r1 = ob.pcg(pb, wp.rhs, 500, 1e-6, tol_mode=1, hierarchy_mode=1)
+    assert (r0.iters, r1.iters) == (35, 12), (r0.iters, r1.iters)

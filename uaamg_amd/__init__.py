@@ -11,7 +11,7 @@ from .capi import (Lib, Context, Batch, Server, Stats, Opts, GridDesc, load, lib
                    ERR_ADMISSION, NOT_CONVERGED, BREAKDOWN, NONFINITE,
                    FORM_ROWS, FORM_ROWS_COARSE, FORM_TAIL, FORM_PMV, FORM_XMASK, FORM_NT_X, FORM_NT_P,
                    FORM_NT_D, FORM_NT_U, FORM_SPLIT, FORM_NT, FORM_DEFAULT, PRECOND_UAAMG, PRECOND_NONE,
-                   PRECOND_GMG, PRECOND_AMG)
+                   PRECOND_GMG, PRECOND_AMG, HIERARCHY_REFERENCE, HIERARCHY_KEEP_FLUID)
 from .build import build, build_host_tools
 
 __all__ = ["Lib", "Context", "Batch", "Server", "Stats", "Opts", "GridDesc", "load", "lib_path", "build", "build_host_tools",
@@ -20,4 +20,4 @@ __all__ = ["Lib", "Context", "Batch", "Server", "Stats", "Opts", "GridDesc", "lo
            "ERR_ADMISSION", "NOT_CONVERGED", "BREAKDOWN", "NONFINITE",
            "FORM_ROWS", "FORM_ROWS_COARSE", "FORM_TAIL", "FORM_PMV", "FORM_XMASK", "FORM_NT_X", "FORM_NT_P",
            "FORM_NT_D", "FORM_NT_U", "FORM_SPLIT", "FORM_NT", "FORM_DEFAULT", "PRECOND_UAAMG", "PRECOND_NONE",
-           "PRECOND_GMG", "PRECOND_AMG"]
+           "PRECOND_GMG", "PRECOND_AMG", "HIERARCHY_REFERENCE", "HIERARCHY_KEEP_FLUID"]

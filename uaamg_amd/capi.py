@@ -17,6 +17,7 @@ FORM_ROWS, FORM_ROWS_COARSE, FORM_TAIL, FORM_PMV, FORM_XMASK = 1, 2, 4, 8, 16
 FORM_NT_X, FORM_NT_P, FORM_NT_D, FORM_NT_U, FORM_SPLIT = 32, 64, 128, 256, 512
 FORM_NT = FORM_NT_X | FORM_NT_P | FORM_NT_D | FORM_NT_U
 PRECOND_UAAMG, PRECOND_NONE, PRECOND_GMG, PRECOND_AMG = 0, 1, 2, 3
+HIERARCHY_REFERENCE, HIERARCHY_KEEP_FLUID = 0, 1
 FORM_DEFAULT = FORM_ROWS | FORM_ROWS_COARSE | FORM_TAIL | FORM_PMV | FORM_NT
 
 
@@ -36,7 +37,8 @@ class Opts(C.Structure):
                 ("fused", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int),
                 ("timing", C.c_int), ("flexible", C.c_int), ("forms", C.c_int), ("xfork_level", C.c_int),
                 ("xblocks", C.c_int), ("xthreads", C.c_int), ("rows_kc", C.c_int), ("march_kc_min", C.c_int),
-                ("march_target", C.c_int), ("trace", C.c_int), ("preconditioner", C.c_int)]
+                ("march_target", C.c_int), ("trace", C.c_int), ("preconditioner", C.c_int),
+                ("hierarchy_mode", C.c_int)]
 
 
 class Stats(C.Structure):

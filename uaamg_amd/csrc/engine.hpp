@@ -250,7 +250,8 @@ public:
         for (int l = 1; l < nl; l++) {
             Level& F = lv_[l - 1]; Level& C = lv_[l];
             const uint8_t* fm = (l == 1) ? nbr0_ : F.mask;
-            k_coarsen_mask<D><<<C.lc.grid, C.lc.block, 0, stream_>>>(fm, C.mask, F.g, C.g, C.lc.kz, l == 1);
+            k_coarsen_mask<D><<<C.lc.grid, C.lc.block, 0, stream_>>>(fm, C.mask, F.g, C.g, C.lc.kz, l == 1,
+                                                                     o_.hierarchy_mode == UAAMG_HIERARCHY_KEEP_FLUID);
             HIPCHK(hipGetLastError());
             if (l == 1) {
                 FineL0 src{nbr0_, c0d_, g0_};

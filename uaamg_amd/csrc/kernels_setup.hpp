@@ -58,7 +58,7 @@ __global__ void k_fin_count(const double* partials, int nb, double* count_out) {
 // solid.  fine_is_nbr: the fine level is level 0 (solid = !(nbr & NB_FLUID)).
 template <int D>
 __global__ void k_coarsen_mask(const uint8_t* __restrict__ fine, uint8_t* __restrict__ coarse,
-                               Geo fg, Geo cg, int kz, int fine_is_nbr) {
+                               Geo fg, Geo cg, int kz, int fine_is_nbr, int all_solid) {
     UAAMG_CELL_LOOP(cg, D, kz) {
         int i_f = 2 * i - 1, j_f = 2 * j - 1, k_f = (D == 3) ? 2 * k - 1 : 0;
         int sc = 0;
@@ -71,7 +71,9 @@ __global__ void k_coarsen_mask(const uint8_t* __restrict__ fine, uint8_t* __rest
                     uint8_t v = fine[gidx(fg, i_f + di, j_f + dj, k_f + dk)];
                     sc += fine_is_nbr ? !(v & NB_FLUID) : (v != 0);
                 }
-        coarse[gidx(cg, i, j, k)] = (sc >= (D == 3 ? 4 : 2)) ? 1 : 0;
+        // the reference's threshold (restrictSolid), or every child solid
+        const int thr = all_solid ? (D == 3 ? 8 : 4) : (D == 3 ? 4 : 2);
+        coarse[gidx(cg, i, j, k)] = (sc >= thr) ? 1 : 0;
     }
 }
 

@@ -211,6 +211,7 @@ extern "C" int uaamg_default_opts(uaamg_opts* o) {
     o->march_target = 1024;
     o->trace = 0;
     o->preconditioner = UAAMG_PRECOND_UAAMG;
+    o->hierarchy_mode = UAAMG_HIERARCHY_REFERENCE;
     return UAAMG_OK;
 }
 
@@ -224,6 +225,7 @@ static bool opts_valid(const uaamg_opts* o, std::string& why) {
     if (o->xfork_level < 0 || o->xblocks < 1 || o->xthreads < 64 || o->xthreads > 1024) { why = "x update launch"; return false; }
     if (o->rows_kc < 0 || o->march_kc_min < 0 || o->march_target < 1) { why = "rows_kc / march_*"; return false; }
     if (o->preconditioner < UAAMG_PRECOND_UAAMG || o->preconditioner > UAAMG_PRECOND_AMG) { why = "preconditioner"; return false; }
+    if (o->hierarchy_mode != UAAMG_HIERARCHY_REFERENCE && o->hierarchy_mode != UAAMG_HIERARCHY_KEEP_FLUID) { why = "hierarchy_mode"; return false; }
     if (o->nu_pre < 0 || o->nu_post < 0 || o->nu_pre > 16 || o->nu_post > 16) { why = "nu_pre/nu_post"; return false; }
     if (o->coarsest_side < 1) { why = "coarsest_side"; return false; }
     if (o->coarsest_sweeps < 0) { why = "coarsest_sweeps"; return false; }
@@ -282,7 +284,7 @@ extern "C" int uaamg_set_opts(uaamg_ctx* c, const uaamg_opts* o) {
     bool rebuild = (o->precision != c->o.precision) || (o->coarsest_side != c->o.coarsest_side) ||
                    ((o->forms ^ c->o.forms) & uaamg::kStructForms) || o->rows_kc != c->o.rows_kc ||
                    o->march_kc_min != c->o.march_kc_min || o->march_target != c->o.march_target ||
-                   o->preconditioner != c->o.preconditioner;
+                   o->preconditioner != c->o.preconditioner || o->hierarchy_mode != c->o.hierarchy_mode;
     c->o = *o;
     if (rebuild) c->eng.reset();     // a grid must be set again
     else if (c->eng) c->eng->set_opts(*o);
