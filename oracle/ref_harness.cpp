@@ -19,6 +19,9 @@
 #include "geometry/cylinder_model.h"
 #include "pressure/pressure.h"
 #include "pressure/pressure_3d.h"
+#include "io/vtk_writer.h"
+#include "io/vtk_writer_3d.h"
+#include "config/config.h"
 #include <memory>
 #include <vector>
 
@@ -174,5 +177,30 @@ extern "C" int ref_project(int dim, int nx, int ny, int nz, double lx, double ly
     for (size_t c = 0; c < g.u.size(); c++) u[c] = g.u[c];
     for (size_t c = 0; c < g.v.size(); c++) v[c] = g.v[c];
     for (size_t c = 0; c < g.w.size(); c++) w[c] = g.w[c];
+    return 0;
+}
+
+// VtkWriter::write / VtkWriter3D::write (src/io/vtk_writer.cpp:9-83,
+// vtk_writer_3d.cpp:10-85) on a grid built from the given MAC arrays and
+// padded solid mask: writes <out_dir>/frame_<frame>.vtk.
+extern "C" int ref_write_vtk(int dim, int nx, int ny, int nz, double lx, double ly, double lz,
+                             const double* u, const double* v, const double* w, const unsigned char* solid,
+                             int frame, const char* out_dir) {
+    Config cfg;
+    cfg.out_dir = out_dir;
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        g.u.assign(u, u + g.u.size());
+        g.v.assign(v, v + g.v.size());
+        for (size_t c = 0; c < g.solid.size(); c++) g.solid[c] = solid[c] != 0;
+        VtkWriter::write(g, frame, cfg);
+    } else {
+        Grid3D g(nx, ny, nz, lx, ly, lz);
+        g.u.assign(u, u + g.u.size());
+        g.v.assign(v, v + g.v.size());
+        g.w.assign(w, w + g.w.size());
+        for (size_t c = 0; c < g.solid.size(); c++) g.solid[c] = solid[c] != 0;
+        VtkWriter3D::write(g, frame, cfg);
+    }
     return 0;
 }

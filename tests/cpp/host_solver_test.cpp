@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "uaamg_solver.hpp"
+#include "uaamg_vtk.hpp"
 
 using namespace uaamg_host;
 
@@ -232,8 +233,47 @@ static int run_solve(const std::string& out) {
     return fails ? 1 : 0;
 }
 
+template <typename T>
+static bool read_raw(const std::string& path, std::vector<T>& out, size_t n) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    out.resize(n);
+    const size_t got = std::fread(out.data(), sizeof(T), n, f);
+    std::fclose(f);
+    return got == n;
+}
+
+// VtkWriter[3D]::write of a grid read from DIR (spec.txt: dim nx ny nz lx ly
+// lz frame; u.f64, v.f64, w.f64 MAC arrays; solid.u8 padded mask) into DIR/ours
+static int run_vtk(const std::string& dir) {
+    FILE* f = std::fopen((dir + "/spec.txt").c_str(), "r");
+    if (!f) return 2;
+    int dim, nx, ny, nz, frame;
+    double lx, ly, lz;
+    const int got = std::fscanf(f, "%d %d %d %d %lf %lf %lf %d", &dim, &nx, &ny, &nz, &lx, &ly, &lz, &frame);
+    std::fclose(f);
+    if (got != 8) return 2;
+    std::vector<uint8_t> s;
+    bool ok;
+    if (dim == 2) {
+        Grid g(nx, ny, lx, ly);
+        ok = read_raw(dir + "/u.f64", g.u, g.u.size()) && read_raw(dir + "/v.f64", g.v, g.v.size()) &&
+             read_raw(dir + "/solid.u8", s, g.solid.size());
+        if (!ok) return 2;
+        for (size_t c = 0; c < s.size(); c++) g.solid[c] = s[c] != 0;
+        return VtkWriter::write(g, frame, dir + "/ours") ? 0 : 3;
+    }
+    Grid3D g(nx, ny, nz, lx, ly, lz);
+    ok = read_raw(dir + "/u.f64", g.u, g.u.size()) && read_raw(dir + "/v.f64", g.v, g.v.size()) &&
+         read_raw(dir + "/w.f64", g.w, g.w.size()) && read_raw(dir + "/solid.u8", s, g.solid.size());
+    if (!ok) return 2;
+    for (size_t c = 0; c < s.size(); c++) g.solid[c] = s[c] != 0;
+    return VtkWriter3D::write(g, frame, dir + "/ours") ? 0 : 3;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && std::strcmp(argv[1], "keys") == 0) return run_keys();
+    if (argc >= 3 && std::strcmp(argv[1], "vtk") == 0) return run_vtk(argv[2]);
     if (argc >= 3 && std::strcmp(argv[1], "solve") == 0) {
         try {
             return run_solve(argv[2]);
@@ -242,6 +282,6 @@ int main(int argc, char** argv) {
             return 2;
         }
     }
-    std::fprintf(stderr, "usage: %s keys | solve OUTDIR\n", argv[0]);
+    std::fprintf(stderr, "usage: %s keys | solve OUTDIR | vtk DIR\n", argv[0]);
     return 64;
 }

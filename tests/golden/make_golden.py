@@ -7,6 +7,8 @@ G1  2D UAAMG V-cycle z = M r   (uaamg_preconditioner.cpp:177-199)
 G2  3D UAAMG V-cycle           (uaamg_preconditioner_3d.cpp:198-229)
 G3  PCG p and preconditioner-application counts (pcg.cpp / pcg_3d.cpp)
 G4  standalone amgpcg x and iteration counts (test/solver/test_amgpcg.cpp)
+G7  VTK frames of VtkWriter / VtkWriter3D (src/io/vtk_writer.cpp, vtk_writer_3d.cpp)
+G6  PCG with the GMG / AMG preconditioners (keys pcg, pcg_gmg, pcg_amg)
 G5  PCG with the identity preconditioner (factory key "cg", factory.cpp:19)
     and with the variable Laplacian of SmoothCylinder::applyToLaplacian
     (cylinder_model.cpp:50-71) that PCG::solve's matvec reads (pcg.cpp:36-46)
@@ -159,9 +161,33 @@ def write_g5(groups=("g5",)):
         print("wrote", name)
 
 
+def write_g7():
+    """VtkWriter / VtkWriter3D frames (src/io/vtk_writer.cpp, vtk_writer_3d.cpp)
+    of seeded MAC fields with a random solid mask; the fixture holds the
+    inputs and the file the reference wrote, as bytes."""
+    import tempfile
+    for name, n, L, seed in (("g7_vtk_2d_6x5", (6, 5), (1.0, 0.8), 1), ("g7_vtk_3d_4x3x3", (4, 3, 3), (1.0, 0.75, 0.6), 2)):
+        rng = np.random.default_rng(seed)
+        shp_u = [n[0] + 1] + [v + 2 for v in n[1:]]
+        shp_v = [n[0] + 2, n[1] + 1] + [v + 2 for v in n[2:]]
+        u = rng.standard_normal(int(np.prod(shp_u)))
+        v = rng.standard_normal(int(np.prod(shp_v)))
+        w = rng.standard_normal((n[0] + 2) * (n[1] + 2) * (n[2] + 1)) if len(n) == 3 else None
+        solid = (rng.random(int(np.prod([x + 2 for x in n]))) < 0.2).astype(np.uint8)
+        with tempfile.TemporaryDirectory() as d:
+            ob.ref_write_vtk(n, L, u, v, w, solid, 12, d)
+            blob = open(os.path.join(d, "frame_00012.vtk"), "rb").read()
+        extra = dict(w=w) if w is not None else {}
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), kind=np.array("vtk"), n=np.array(n), L=np.array(L),
+                            u=u, v=v, solid=solid, frame=np.array(12), vtk=np.frombuffer(blob, np.uint8), **extra)
+        print("wrote", name, len(blob), "bytes")
+
+
 def main(groups):
     assert ob.ref_available(), "build oracle/_ref first: make -C oracle ref"
     os.makedirs(HERE, exist_ok=True)
+    if "g7" in groups:
+        write_g7()
     if {"g5", "g6"} & set(groups):
         write_g5(groups)
     if not ({"g1", "g2", "g3", "g4"} & set(groups)):
@@ -198,5 +224,5 @@ def main(groups):
 
 
 if __name__ == "__main__":
-    # python make_golden.py [g1 g2 g3 g4 g5 g6]  (default: all groups)
-    main(sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6"])
+    # python make_golden.py [g1 g2 g3 g4 g5 g6 g7]  (default: all groups)
+    main(sys.argv[1:] or ["g1", "g2", "g3", "g4", "g5", "g6", "g7"])

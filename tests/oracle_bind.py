@@ -77,6 +77,7 @@ def ref():
         _ref.ref_project.argtypes = [I, I, I, I, D, D, D, P, P, P, P, P, D, I, D]
         _ref.ref_pcg_ex.argtypes = [I, I, I, I, D, D, D, P, P, P, I, D, I, P, P, P]
         _ref.ref_smooth_cylinder_lap.argtypes = [I, I, D, D, D, D, D, D, P, P, P, P]
+        _ref.ref_write_vtk.argtypes = [I, I, I, I, D, D, D, P, P, P, P, I, C.c_char_p]
     return _ref
 
 
@@ -246,6 +247,18 @@ def ref_pcg_ex(pb, rhs, max_iter, tol, precond=PRECOND_UAAMG, p0=None):
                           max_iter, tol, precond, *[(_p(a) if a is not None else None) for a in lap])
     assert rc == 0, rc
     return p
+
+
+def ref_write_vtk(n, L, u, v, w, solid, frame, out_dir):
+    """The reference's VtkWriter[3D]::write (vtk_writer.cpp, vtk_writer_3d.cpp)
+    of a grid with these MAC arrays and padded solid mask into out_dir."""
+    dim = len(n)
+    nz = n[2] if dim == 3 else 1
+    lz = L[2] if dim == 3 else 1.0
+    u = np.ascontiguousarray(u, np.float64); v = np.ascontiguousarray(v, np.float64)
+    w = np.ascontiguousarray(w if w is not None else np.zeros(1), np.float64)
+    s = np.ascontiguousarray(solid, np.uint8)
+    ref().ref_write_vtk(dim, n[0], n[1], nz, L[0], L[1], lz, _p(u), _p(v), _p(w), _p(s), frame, out_dir.encode())
 
 
 def ref_smooth_cylinder_lap(n, L, centre, R, tw=0.0):
