@@ -193,7 +193,7 @@ public:
         gmg_ = o.preconditioner == UAAMG_PRECOND_GMG ? 1 : (o.preconditioner == UAAMG_PRECOND_AMG ? 2 : 0);
         if (gmg_ && (D != 2 || sizeof(S) < 4))
             throw HipError{hipErrorInvalidValue, "GMG / AMG preconditioners: 2D grids, FP64 or FP32 levels"};
-        if (gmg_) gmg_setup();
+        if (gmg_) { gmg_setup(); gmg_tail_setup(); }
         if (!ident_ && !gmg_) rows_setup(h, wall);   // the identity and GMG / AMG forms run the flat kernels only
         timer_.s = stream_;
         if (pmv_on_) {   // the x update's stream, created before any capture
@@ -353,70 +353,84 @@ public:
         using X = typename CTOf<V>::type;
         return GmgC<X>{(X)gmgc_[l].idx2, (X)gmgc_[l].idy2, (X)gmgc_[l].diag};
     }
-    static dim3 gmg_grid(const Geo& g) { return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4)); }
-    template <typename V>
-    void gmg_colour(V* x, const V* b, const uint8_t* m, int l, int odd, const int* skip) {
-        const Geo& g = shape_[l];
-        k_gmg_colour<V><<<gmg_grid(g), dim3(64, 4), 0, stream_>>>(x, b, m, g, gmg_c<V>(l), odd, skip);
-        HIPCHK(hipGetLastError());
-    }
-    // smooth (odd colour first) or smoothReverse (even first), `sweeps` times
-    template <typename V>
-    void gmg_smooth(V* x, const V* b, const uint8_t* m, int l, int sweeps, bool reverse, const int* skip) {
-        for (int s = 0; s < sweeps; s++) {
-            gmg_colour<V>(x, b, m, l, reverse ? 0 : 1, skip);
-            gmg_colour<V>(x, b, m, l, reverse ? 1 : 0, skip);
+    // levels from gmg_lt_ on run in one workgroup (k_gmg_tail), the ones
+    // above it with one tiled launch down and one up per level
+    int gmg_lt_ = 0;
+    GmgTailArgs gmg_targs_{};
+    size_t gmg_tail_lds_ = 0;
+    void gmg_tail_setup() {
+        const int nl = nlevels();
+        const size_t cb = (sizeof(T) == 8 || sizeof(S) == 8) ? 8 : 4;   // LDS element size, the larger form
+        size_t bytes = 0;
+        int lt = nl;
+        while (lt > 0 && nl - (lt - 1) <= GMG_TAIL_MAXL) {
+            const size_t add = (size_t)shape_[lt - 1].N * (2 * cb + 1);
+            if (bytes + add > (size_t)GMG_TAIL_LDS) break;
+            bytes += add;
+            lt--;
         }
+        if (lt == nl)   // a coarsest level of side <= coarsest_side in one dimension only, long in the other
+            throw HipError{hipErrorInvalidValue, "GMG / AMG: coarsest level too large for the one-workgroup solve"};
+        gmg_lt_ = lt;
     }
-    template <typename VF>
-    void gmg_restrict(const VF* x, const VF* b, const uint8_t* m, int l, const int* skip) {
+    void gmg_tail_args(int csweeps) {
+        const int nl = nlevels();
+        GmgTailArgs& A = gmg_targs_;
+        A.nl = nl - gmg_lt_;
+        int off = 0;
+        for (int q = 0; q < A.nl; q++) {
+            const int l = gmg_lt_ + q;
+            A.g[q] = shape_[l];
+            A.c[q] = gmgc_[l];
+            A.mask[q] = l == 0 ? solid0_ : lv_[l].mask;
+            A.off[q] = off;
+            off += (int)shape_[l].N;
+        }
+        A.total = off;
+        A.csweeps = csweeps;
+        const size_t cb = gmg_lt_ == 0 ? sizeof(typename CTOf<T>::type) : sizeof(typename CTOf<S>::type);
+        gmg_tail_lds_ = (size_t)off * (2 * cb + 1);
+    }
+    static dim3 gmg_tiles(const Geo& g) { return dim3((unsigned)((g.nx + GT_X - 1) / GT_X), (unsigned)((g.ny + GT_Y - 1) / GT_Y)); }
+    template <typename VX>
+    void gmg_down(VX* x, const VX* b, const uint8_t* m, int l, const int* skip) {
         Level& C = lv_[l + 1];
-        auto kern = gmg_ == 2 ? k_gmg_restrict<true, VF, S> : k_gmg_restrict<false, VF, S>;
-        kern<<<gmg_grid(C.g), dim3(64, 4), 0, stream_>>>(x, b, m, shape_[l], gmg_c<VF>(l), C.b, C.mask, C.g, skip);
+        auto kern = gmg_ == 2 ? k_gmg_down<true, VX, S> : k_gmg_down<false, VX, S>;
+        kern<<<gmg_tiles(shape_[l]), GT_THREADS, 0, stream_>>>(x, b, m, shape_[l], gmg_c<VX>(l), C.b, C.mask, C.g, skip);
+        HIPCHK(hipGetLastError());
+    }
+    template <typename VX>
+    void gmg_up(VX* x, const VX* b, const uint8_t* m, int l, const int* skip) {
+        Level& C = lv_[l + 1];
+        auto kern = gmg_ == 2 ? k_gmg_up<true, VX, S> : k_gmg_up<false, VX, S>;
+        kern<<<gmg_tiles(shape_[l]), GT_THREADS, 0, stream_>>>(x, b, m, shape_[l], gmg_c<VX>(l), C.x, C.mask, C.g, skip);
         HIPCHK(hipGetLastError());
     }
     template <typename VF>
-    void gmg_prolong(VF* x, const uint8_t* m, int l, const int* skip) {
-        Level& C = lv_[l + 1];
-        auto kern = gmg_ == 2 ? k_gmg_prolong<true, VF, S> : k_gmg_prolong<false, VF, S>;
-        kern<<<gmg_grid(shape_[l]), dim3(64, 4), 0, stream_>>>(x, m, shape_[l], C.x, C.mask, C.g, skip);
+    void gmg_tail(VF* x, const VF* b, const int* skip) {
+        gmg_tail_args(std::max(0, o_.coarsest_sweeps));
+        auto kern = gmg_ == 2 ? k_gmg_tail<true, VF> : k_gmg_tail<false, VF>;
+        kern<<<1, GMG_TAIL_THREADS, gmg_tail_lds_, stream_>>>(gmg_targs_, x, b, skip);
         HIPCHK(hipGetLastError());
     }
+    // z = M r: levels 0 .. lt-1 down (x of each level and b of the next
+    // written), the tail from lt, levels lt-1 .. 0 up; every cell of every
+    // x the V-cycle reads is written earlier in the same V-cycle, so nothing
+    // is zero-filled
     void vcycle_gmg(const T* r, T* z, const int* skip) {
         if constexpr (D == 2 && sizeof(S) >= 4) {
             TScope ts(timer_, UAAMG_K_DOWN0);
-            const int nl = nlevels();
-            fill_async(z, g0_.N, skip);
-            for (int l = 1; l < nl; l++) fill_async(lv_[l].x, lv_[l].g.N, skip);
-            const int cs = std::max(0, o_.coarsest_sweeps);
-            if (nl == 1) {
-                for (int s = 0; s < cs; s++) {
-                    gmg_smooth<T>(z, r, solid0_, 0, 1, false, skip);
-                    gmg_smooth<T>(z, r, solid0_, 0, 1, true, skip);
-                }
-                return;
+            const int lt = gmg_lt_;
+            for (int l = 0; l < lt; l++) {
+                if (l == 0) gmg_down<T>(z, r, solid0_, 0, skip);
+                else gmg_down<S>(lv_[l].x, lv_[l].b, lv_[l].mask, l, skip);
             }
-            gmg_smooth<T>(z, r, solid0_, 0, 2, false, skip);
-            gmg_restrict<T>(z, r, solid0_, 0, skip);
-            for (int l = 1; l < nl - 1; l++) {
-                Level& L = lv_[l];
-                gmg_smooth<S>(L.x, L.b, L.mask, l, 2, false, skip);
-                gmg_restrict<S>(L.x, L.b, L.mask, l, skip);
+            if (lt == 0) gmg_tail<T>(z, r, skip);
+            else gmg_tail<S>(lv_[lt].x, lv_[lt].b, skip);
+            for (int l = lt - 1; l >= 0; l--) {
+                if (l == 0) gmg_up<T>(z, r, solid0_, 0, skip);
+                else gmg_up<S>(lv_[l].x, lv_[l].b, lv_[l].mask, l, skip);
             }
-            {
-                Level& L = lv_[nl - 1];
-                for (int s = 0; s < cs; s++) {
-                    gmg_smooth<S>(L.x, L.b, L.mask, nl - 1, 1, false, skip);
-                    gmg_smooth<S>(L.x, L.b, L.mask, nl - 1, 1, true, skip);
-                }
-            }
-            for (int l = nl - 2; l >= 1; l--) {
-                Level& L = lv_[l];
-                gmg_prolong<S>(L.x, L.mask, l, skip);
-                gmg_smooth<S>(L.x, L.b, L.mask, l, 2, true, skip);
-            }
-            gmg_prolong<T>(z, solid0_, 0, skip);
-            gmg_smooth<T>(z, r, solid0_, 0, 2, true, skip);
         }
     }
 
