@@ -28,6 +28,42 @@ __device__ __forceinline__ long gidx(const Geo& g, int i, int j, int k) {
     return (long)i + (long)j * g.sx + (long)k * g.sy;
 }
 
+// Load / store through a pointer the compiler cannot prove to be global (one
+// read back from LDS or from a descriptor in memory).  Through a generic
+// pointer it emits flat instructions, which also count against the LDS wait
+// counter: every later LDS wait would then wait for memory as well.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    if constexpr (sizeof(T) == 32) {
+        struct R { u4 a, b; } r;
+        r.a = *(const __attribute__((address_space(1))) u4*)p;
+        r.b = *((const __attribute__((address_space(1))) u4*)p + 1);
+        return __builtin_bit_cast(T, r);
+    } else if constexpr (sizeof(T) == 16) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) u4*)p);
+    } else if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) u2*)p);
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) unsigned*)p);
+    } else {
+        static_assert(sizeof(T) == 2, "gload: 2, 4, 8, 16 or 32 bytes");
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) unsigned short*)p);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+    if constexpr (sizeof(T) == 8) {
+        *(__attribute__((address_space(1))) unsigned long long*)p = __builtin_bit_cast(unsigned long long, v);
+    } else if constexpr (sizeof(T) == 4) {
+        *(__attribute__((address_space(1))) unsigned*)p = __builtin_bit_cast(unsigned, v);
+    } else {
+        static_assert(sizeof(T) == 2, "gstore: 2, 4 or 8 bytes");
+        *(__attribute__((address_space(1))) unsigned short*)p = __builtin_bit_cast(unsigned short, v);
+    }
+}
+
 // Level-0 neighbour byte: which couplings of the fine stencil exist.
 // bit0 fluid, bit1 +x, bit2 -x, bit3 +y, bit4 -y, bit5 +z, bit6 -z.
 // A coupling exists iff both cells are fluid and the face is interior

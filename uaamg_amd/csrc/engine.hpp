@@ -263,6 +263,7 @@ public:
             k_level_diag<D, C4, S><<<C.lc.grid, C.lc.block, 0, stream_>>>(C.coef, C.ex, C.mask, C.g, C.lc.kz);
             HIPCHK(hipGetLastError());
         }
+        tail_build();
         destroy_graph();
     }
 
@@ -494,6 +495,7 @@ public:
             d.rgeo = rgeo_;
             d.tail = targs_;
             d.tail.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+            d.tail.skip = &ctrl->done;
         }
     }
 
@@ -502,8 +504,9 @@ public:
     int tail_first_ = 0;        // first level of the tail (0: no tail)
     TailArgs<C4> targs_{};
     // First level of the tail kernel for a hierarchy (0: none fits): the
-    // largest tail whose first level has <= 2048 cell pairs, deeper levels
-    // <= 1024, at most TAIL_NLMAX levels and x, b, residual within the LDS.
+    // largest tail whose first level has <= 2048 cell pairs and <= 8192
+    // padded cells, deeper levels <= 1024 pairs, at most TAIL_NLMAX levels and
+    // x, b, residual within the LDS.
     static int choose_tail(const std::vector<Geo>& shape) {
         const int nl = (int)shape.size();
         auto pairs = [&](int l) {
@@ -513,6 +516,7 @@ public:
         for (int lt = 1; lt <= nl - 1; lt++) {
             if (nl - lt > TAIL_NLMAX) continue;
             if (pairs(lt) > (long)TAIL_MAXP * TAIL_THREADS) continue;
+            if (shape[lt].N > (long)TAIL_BMAX * TAIL_THREADS) continue;
             bool fit = true;
             for (int l = lt + 1; l < nl; l++) fit = fit && pairs(l) <= TAIL_THREADS;
             long elems = 0;
@@ -525,7 +529,7 @@ public:
     }
     void tail_setup() {
         tail_first_ = 0;
-        if (!(o_.forms & UAAMG_FORM_TAIL)) return;
+        if (!(o_.forms & UAAMG_FORM_TAIL) || ident_ || gmg_) return;
         const int nl = nlevels();
         tail_first_ = choose_tail(shape_);
         if (!tail_first_) return;
@@ -540,11 +544,25 @@ public:
         }
         targs_.ro = off;
         targs_.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
+        targs_.b = lv_[tail_first_].b;
+        targs_.x = lv_[tail_first_].x;
+        targs_.skip = nullptr;
+        // per-thread stencil records (filled by tail_build once the level
+        // records exist)
+        tail_rec_ = dalloc<char>((size_t)tail_rec_bytes<SC>(TAIL_MAXP + targs_.nl - 1));
+        targs_.rec = tail_rec_;
+    }
+    char* tail_rec_ = nullptr;
+    // after the level records of a grid are built
+    void tail_build() {
+        if (!tail_first_) return;
+        k_tail_records<D, SC, C4><<<TAIL_MAXP + targs_.nl - 1, TAIL_THREADS, 0, stream_>>>(targs_, reinterpret_cast<SC*>(tail_rec_));
+        HIPCHK(hipGetLastError());
     }
     void tail_launch(const int* skip) {
         targs_.npass = (o_.coarsest_sweeps > 0) ? 2 * o_.coarsest_sweeps + 1 : 0;
-        Level& TL = lv_[tail_first_];
-        k_tail<D, S, C4, TAIL_NLMAX><<<1, TAIL_THREADS, 0, stream_>>>(targs_, TL.b, TL.x, skip);
+        targs_.skip = skip;
+        k_tail<D, S, C4, TAIL_NLMAX><<<1, TAIL_THREADS, 0, stream_>>>(targs_);
     }
 
     // ---- level-0 row kernels (3D, FP32 vectors, nx <= 256) ----

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void kb_march_up(BGrid<S>* G, const BItem* ite
 template <int D, typename S, int NLMAX>
 __global__ __launch_bounds__(TAIL_THREADS) void kb_tail(BGrid<S>* G) {
     const BGrid<S>& d = G[blockIdx.x];
-    d_tail<D, S, typename Vec4<S>::type, NLMAX>(d.tail, d.lb[d.lt], d.lx[d.lt], &d.ctrl->done);
+    d_tail<D, S, typename Vec4<S>::type, NLMAX>(d.tail);
 }
 
 }  // namespace uaamg

@@ -24,6 +24,19 @@ constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_MAXL = 5;      // tail levels (3D: 16^3..4^3 is 3; 2D: 64^2..4^2 is 5)
 constexpr int TAIL_MAXP = 2;      // pairs per thread on the first tail level
 constexpr int TAIL_LDS_BYTES = 96 * 1024;   // x, b of every tail level + one residual array
+constexpr int TAIL_BMAX = 8;     // b elements per thread of the first tail level (padded cells <= 8192)
+constexpr int TAIL_NSLOT = TAIL_MAXP + TAIL_MAXL - 1;   // record slots: TAIL_MAXP on the first level, 1 below
+
+// Stencil records of the owned pairs, built at set-up in the compute type:
+// 14 values per pair (d, xp, xm, yp, ym, zp, zm of both cells), NQ 16-byte
+// quads per thread, stored [slot][quad][thread] so that every quad load of a
+// wave is one contiguous 1 KiB read.
+template <typename CT> struct TailRec;
+template <> struct TailRec<float> { static constexpr int NQ = 4; using Q = float4; };
+template <> struct TailRec<double> { static constexpr int NQ = 7; using Q = double2; };
+template <typename CT> constexpr int tail_rec_per_q() { return 16 / (int)sizeof(CT); }
+template <typename CT> constexpr long tail_rec_bytes(int nslots) { return (long)nslots * TailRec<CT>::NQ * TAIL_THREADS * 16; }
+__host__ __device__ constexpr int tail_slot(int l, int v) { return l == 0 ? v : TAIL_MAXP + l - 1; }
 
 template <typename C4>
 struct TailArgs {
@@ -33,6 +46,10 @@ struct TailArgs {
     int xo[TAIL_MAXL], bo[TAIL_MAXL];   // LDS offsets (elements) of x and b per level
     int ro;                        // LDS offset of the residual scratch
     int npass;                     // coarsest passes (2 * sweeps + 1)
+    const void* b;                 // b of the first tail level (level storage type)
+    void* x;                       // x of the first tail level (written at the end)
+    const int* skip;               // stop flag (nullptr: never skip)
+    const void* rec;               // pair records (compute type), TAIL_NSLOT slots at most
 };
 
 // Stencils of one owned pair of cells, in the reference channel order.
@@ -45,8 +62,46 @@ struct TailPair {
     bool valid, ok1;  // pair exists; second cell inside the level
 };
 
+// Record element type of a coefficient vector type.
+template <typename C4> struct TailE { using E = float; };
+template <> struct TailE<double4> { using E = double; };
+template <> struct TailE<BF16x4> { using E = __bf16; };
+
+// The record loads of one pair as they come from memory.  Every pair of
+// every tail level issues its loads before any of them is used, so that the
+// start of the kernel waits for one memory round trip instead of one per
+// pair; an absent pair reads the first interior cell (in bounds, discarded)
+// and the second cell of a pair at i = nx reads the ghost at nx + 1 (inside
+// the padded level, discarded).
+template <typename C4>
+struct TailRaw {
+    C4 c[2];
+    typename TailE<C4>::E xm[2], ym[2], zm[2];
+};
+
+template <int D, typename C4>
+__device__ __forceinline__ void tail_issue_pair(TailRaw<C4>& R, const C4* __restrict__ coef, const Geo& g, int p) {
+    const int npx = (g.nx + 1) >> 1;
+    const int npairs = npx * g.ny * (D == 3 ? g.nz : 1);
+    const int pp = p < npairs ? p : 0;
+    const int ip = pp % npx;
+    const int j = (pp / npx) % g.ny + 1, k = (D == 3) ? pp / (npx * g.ny) + 1 : 0;
+    const int id0 = (int)gidx(g, 2 * ip + 1, j, k);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int id = id0 + q;
+        using E = typename TailE<C4>::E;
+        const E* e = reinterpret_cast<const E*>(coef);   // {x, y, z, w} per record
+        R.c[q] = gload(coef + id);
+        R.xm[q] = gload(e + 4 * (id - 1));
+        R.ym[q] = gload(e + 4 * (id - g.sx) + 1);
+        if (D == 3) R.zm[q] = gload(e + 4 * (id - g.sy) + 2);
+    }
+}
+
+// The stencils of pair p from its loaded records, as sten_lc forms them.
 template <int D, typename CT, typename C4>
-__device__ __forceinline__ void tail_load_pair(TailPair<CT>& P, const C4* __restrict__ coef, const Geo& g, int p) {
+__device__ __forceinline__ void tail_load_pair(TailPair<CT>& P, const TailRaw<C4>& R, const Geo& g, int p) {
     const int npx = (g.nx + 1) >> 1;
     const int npairs = npx * g.ny * (D == 3 ? g.nz : 1);
     P.valid = p < npairs;
@@ -60,14 +115,86 @@ __device__ __forceinline__ void tail_load_pair(TailPair<CT>& P, const C4* __rest
     P.id0 = (int)gidx(g, P.i0, P.j, P.k);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-        const long id = P.id0 + q;
         if (q == 1 && !P.ok1) {
             P.d[q] = CT(-1); P.xp[q] = P.xm[q] = P.yp[q] = P.ym[q] = P.zp[q] = P.zm[q] = CT(0);
             continue;
         }
-        Sten<CT> s = sten_lc<D, CT>(coef, g, id);
-        P.d[q] = s.d; P.xp[q] = s.xp; P.xm[q] = s.xm; P.yp[q] = s.yp; P.ym[q] = s.ym; P.zp[q] = s.zp; P.zm[q] = s.zm;
+        const C4 c0 = R.c[q];
+        CT d = (CT)c0.w;
+        P.xp[q] = (CT)c0.x; P.yp[q] = (CT)c0.y;
+        P.xm[q] = (CT)R.xm[q]; P.ym[q] = (CT)R.ym[q];
+        if (D == 3) { P.zp[q] = (CT)c0.z; P.zm[q] = (CT)R.zm[q]; }
+        else { P.zp[q] = CT(0); P.zm[q] = CT(0); }
+        if (d >= CT(0)) d = snap_diag<C4, CT>(d, P.xp[q] + P.xm[q] + P.yp[q] + P.ym[q] + P.zp[q] + P.zm[q]);
+        P.d[q] = d;
     }
+}
+
+// The 14 record values of one pair, as loaded.
+template <typename CT>
+struct TailRecV { CT v[TailRec<CT>::NQ * tail_rec_per_q<CT>()]; };
+
+template <typename CT>
+__device__ __forceinline__ void tail_issue_rec(TailRecV<CT>& R, const void* rec, int slot, int t) {
+    using Q = typename TailRec<CT>::Q;
+    constexpr int NQ = TailRec<CT>::NQ;
+    const Q* base = static_cast<const Q*>(rec) + (long)slot * NQ * TAIL_THREADS + t;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const Q x = gload(base + q * TAIL_THREADS);
+        if constexpr (sizeof(CT) == 4) { R.v[4 * q] = x.x; R.v[4 * q + 1] = x.y; R.v[4 * q + 2] = x.z; R.v[4 * q + 3] = x.w; }
+        else { R.v[2 * q] = x.x; R.v[2 * q + 1] = x.y; }
+    }
+}
+
+// Geometry of pair p (computed) and its stencils (from the record).
+template <int D, typename CT>
+__device__ __forceinline__ void tail_pair_from_rec(TailPair<CT>& P, const TailRecV<CT>& R, const Geo& g, int p) {
+    const int npx = (g.nx + 1) >> 1;
+    const int npairs = npx * g.ny * (D == 3 ? g.nz : 1);
+    P.valid = p < npairs;
+    const int pp = P.valid ? p : 0;
+    const int ip = pp % npx;
+    P.i0 = 2 * ip + 1;
+    P.j = (pp / npx) % g.ny + 1;
+    P.k = (D == 3) ? pp / (npx * g.ny) + 1 : 0;
+    P.ok1 = P.valid && P.i0 + 1 <= g.nx;
+    P.par0 = (P.i0 + P.j + P.k) & 1;
+    P.id0 = (int)gidx(g, P.i0, P.j, P.k);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        P.d[q] = R.v[q]; P.xp[q] = R.v[2 + q]; P.xm[q] = R.v[4 + q]; P.yp[q] = R.v[6 + q];
+        P.ym[q] = R.v[8 + q]; P.zp[q] = R.v[10 + q]; P.zm[q] = R.v[12 + q];
+    }
+}
+
+// Set-up: the records of every slot of the tail (one workgroup per slot).
+template <int D, typename CT, typename C4>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail_records(TailArgs<C4> A, CT* __restrict__ rec) {
+    const int slot = blockIdx.x;
+    const int l = slot < TAIL_MAXP ? 0 : slot - TAIL_MAXP + 1;
+    const int v = slot < TAIL_MAXP ? slot : 0;
+    if (l >= A.nl) return;
+    const int t = threadIdx.x, p = t + TAIL_THREADS * v;
+    TailRaw<C4> R;
+    tail_issue_pair<D, C4>(R, A.coef[l], A.g[l], p);
+    TailPair<CT> P;
+    tail_load_pair<D, CT, C4>(P, R, A.g[l], p);
+    constexpr int PER = tail_rec_per_q<CT>(), NQ = TailRec<CT>::NQ;
+    CT f[NQ * PER];
+#pragma unroll
+    for (int e = 0; e < NQ * PER; e++) f[e] = CT(0);
+    if (P.valid) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            f[q] = P.d[q]; f[2 + q] = P.xp[q]; f[4 + q] = P.xm[q]; f[6 + q] = P.yp[q];
+            f[8 + q] = P.ym[q]; f[10 + q] = P.zp[q]; f[12 + q] = P.zm[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+        for (int e = 0; e < PER; e++) rec[(((long)slot * NQ + q) * TAIL_THREADS + t) * PER + e] = f[q * PER + e];
 }
 
 // (b + nb) / d: exact division in FP64 (the parity path); in FP32 the
@@ -166,34 +293,77 @@ __device__ __forceinline__ void tail_restrict(const CT* Rs, CT* BC, CT* XC, cons
 }
 
 template <int D, typename S, typename C4, int NLMAX>
-__device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restrict__ b_first, S* __restrict__ x_first,
-                                       const int* skip) {
+__device__ __forceinline__ void d_tail(const TailArgs<C4>& Ag) {
     static_assert(NLMAX >= 2 && NLMAX <= TAIL_MAXL, "tail depth");
     using CT = typename CTOf<S>::type;
-    if (skip && *skip) return;
-    __shared__ CT lds[TAIL_LDS_BYTES / sizeof(CT)];
+    __shared__ __align__(16) CT lds[TAIL_LDS_BYTES / sizeof(CT)];
     const int tid = threadIdx.x;
+    // The arguments sit in memory no cache has seen yet (a fresh kernel
+    // argument block, or the batch descriptor): scalar loads would fetch them
+    // line by line as the code first touches them, one memory round trip
+    // each.  One vector load per dword brings the block in one round trip.
+    // The skip flag is read beside the first data loads and tested before
+    // the first store to memory, not waited for up front.
+    __shared__ TailArgs<C4> A;
+    {
+        constexpr int NW = (int)(sizeof(TailArgs<C4>) / 4);
+        static_assert(NW <= TAIL_THREADS && sizeof(TailArgs<C4>) % 4 == 0, "tail arguments");
+        if (tid < NW) reinterpret_cast<uint32_t*>(&A)[tid] = reinterpret_cast<const uint32_t*>(&Ag)[tid];
+    }
+    __syncthreads();
+    const S* __restrict__ b_first = static_cast<const S*>(A.b);
+    S* __restrict__ x_first = static_cast<S*>(A.x);
     const int nl = A.nl;   // 1 <= nl <= NLMAX (checked on the host); 1: the coarsest level only
 
-    // registers: the first tail level has up to 2 pairs per thread, the others 1
-    TailPair<CT> P0[TAIL_MAXP], P1[1], P2[1], P3[1], P4[1];
+    // every global load of the kernel's start first: the records of the
+    // owned pairs (the first tail level has up to 2 pairs per thread, the
+    // others 1) and b of the first level (at most TAIL_BMAX per thread,
+    // checked on the host)
+    TailRecV<CT> R0[TAIL_MAXP], R1, R2, R3, R4;
 #pragma unroll
-    for (int v = 0; v < TAIL_MAXP; v++) tail_load_pair<D, CT, C4>(P0[v], A.coef[0], A.g[0], tid + TAIL_THREADS * v);
-    if (nl > 1) tail_load_pair<D, CT, C4>(P1[0], A.coef[1], A.g[1], tid); else P1[0].valid = false;
-    if constexpr (NLMAX > 2) { if (nl > 2) tail_load_pair<D, CT, C4>(P2[0], A.coef[2], A.g[2], tid); }
-    if constexpr (NLMAX > 3) { if (nl > 3) tail_load_pair<D, CT, C4>(P3[0], A.coef[3], A.g[3], tid); }
-    if constexpr (NLMAX > 4) { if (nl > 4) tail_load_pair<D, CT, C4>(P4[0], A.coef[4], A.g[4], tid); }
+    for (int v = 0; v < TAIL_MAXP; v++) tail_issue_rec<CT>(R0[v], A.rec, tail_slot(0, v), tid);
+    if (nl > 1) tail_issue_rec<CT>(R1, A.rec, tail_slot(1, 0), tid);
+    if constexpr (NLMAX > 2) { if (nl > 2) tail_issue_rec<CT>(R2, A.rec, tail_slot(2, 0), tid); }
+    if constexpr (NLMAX > 3) { if (nl > 3) tail_issue_rec<CT>(R3, A.rec, tail_slot(3, 0), tid); }
+    if constexpr (NLMAX > 4) { if (nl > 4) tail_issue_rec<CT>(R4, A.rec, tail_slot(4, 0), tid); }
+    S bt[TAIL_BMAX];
+    const int N0 = (int)A.g[0].N;
+#pragma unroll
+    for (int u = 0; u < TAIL_BMAX; u++) {
+        const int e = tid + TAIL_THREADS * u;
+        if (e < N0) bt[u] = gload(b_first + e);
+    }
+    // a vector load (the index lives in a VGPR), so that the flag is waited
+    // for where it is tested, not at once by a scalar branch
+    int skipped = 0;
+    if (A.skip) {
+        int z = 0;
+        asm volatile("" : "+v"(z));
+        skipped = gload(A.skip + z);
+    }
 
-    // x = 0 on every level (ghosts included); b of the first level from memory
+    // x = 0 on every level (ghosts included) while the loads are in flight
     for (int l = 0; l < nl; l++) {
         CT* X = lds + A.xo[l];
         for (int e = tid; e < A.g[l].N; e += TAIL_THREADS) X[e] = CT(0);
     }
+    TailPair<CT> P0[TAIL_MAXP], P1[1], P2[1], P3[1], P4[1];
+#pragma unroll
+    for (int v = 0; v < TAIL_MAXP; v++) tail_pair_from_rec<D, CT>(P0[v], R0[v], A.g[0], tid + TAIL_THREADS * v);
+    if (nl > 1) tail_pair_from_rec<D, CT>(P1[0], R1, A.g[1], tid); else P1[0].valid = false;
+    if constexpr (NLMAX > 2) { if (nl > 2) tail_pair_from_rec<D, CT>(P2[0], R2, A.g[2], tid); }
+    if constexpr (NLMAX > 3) { if (nl > 3) tail_pair_from_rec<D, CT>(P3[0], R3, A.g[3], tid); }
+    if constexpr (NLMAX > 4) { if (nl > 4) tail_pair_from_rec<D, CT>(P4[0], R4, A.g[4], tid); }
     {
         CT* B0 = lds + A.bo[0];
-        for (int e = tid; e < A.g[0].N; e += TAIL_THREADS) B0[e] = (CT)b_first[e];
+#pragma unroll
+        for (int u = 0; u < TAIL_BMAX; u++) {
+            const int e = tid + TAIL_THREADS * u;
+            if (e < N0) B0[e] = (CT)bt[u];
+        }
     }
     __syncthreads();
+    if (skipped) return;   // uniform: every thread read the same flag
 
     auto level_down = [&](auto& P, int l) {
         CT* X = lds + A.xo[l];
@@ -262,14 +432,13 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& A, const S* __restric
     if (nl > 1) level_up(P0, 0);
     {
         const CT* X0 = lds + A.xo[0];
-        for (int e = tid; e < A.g[0].N; e += TAIL_THREADS) x_first[e] = (S)X0[e];
+        for (int e = tid; e < A.g[0].N; e += TAIL_THREADS) gstore(x_first + e, (S)X0[e]);
     }
 }
 
 template <int D, typename S, typename C4, int NLMAX>
-__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs<C4> A, const S* __restrict__ b_first,
-                                                        S* __restrict__ x_first, const int* skip) {
-    d_tail<D, S, C4, NLMAX>(A, b_first, x_first, skip);
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs<C4> A) {
+    d_tail<D, S, C4, NLMAX>(A);
 }
 
 }  // namespace uaamg

@@ -632,6 +632,17 @@ extern "C" size_t uaamg_bytes_required(const uaamg_grid* g, const uaamg_opts* o)
         }
     }
     bytes += (3 * (size_t)maxb + 64) * 8;
+    // the tail kernel's pair records (Engine::tail_setup)
+    if ((o->forms & UAAMG_FORM_TAIL) && o->preconditioner == UAAMG_PRECOND_UAAMG) {
+        const bool l64 = sl == 8;
+        int lt;
+        if (g->dim == 3) lt = l64 ? Engine<3, double, double>::choose_tail(shape) : Engine<3, float, float>::choose_tail(shape);
+        else lt = l64 ? Engine<2, double, double>::choose_tail(shape) : Engine<2, float, float>::choose_tail(shape);
+        if (lt > 0) {
+            const int nslots = uaamg::TAIL_MAXP + (int)shape.size() - lt - 1;
+            bytes += (size_t)(l64 ? uaamg::tail_rec_bytes<double>(nslots) : uaamg::tail_rec_bytes<float>(nslots));
+        }
+    }
     return bytes;
 }
 
