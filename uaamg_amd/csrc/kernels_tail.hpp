@@ -366,22 +366,29 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& Ag) {
     if (skipped) return;   // uniform: every thread read the same flag
 
     auto level_down = [&](auto& P, int l) {
+        // geometry and offsets copied to registers once: read through the
+        // LDS copy of the arguments they would be fetched again after every
+        // barrier
+        const Geo g = A.g[l], gc = A.g[l + 1];
         CT* X = lds + A.xo[l];
         const CT* Bv = lds + A.bo[l];
         CT* Rs = lds + A.ro;
-        tail_smooth<D>(P, X, Bv, A.g[l], 1);
+        CT* BC = lds + A.bo[l + 1];
+        CT* XC = lds + A.xo[l + 1];
+        tail_smooth<D>(P, X, Bv, g, 1);
         __syncthreads();
-        tail_smooth<D>(P, X, Bv, A.g[l], 0);
+        tail_smooth<D>(P, X, Bv, g, 0);
         __syncthreads();
-        tail_residual<D>(P, X, Bv, Rs, A.g[l]);
+        tail_residual<D>(P, X, Bv, Rs, g);
         __syncthreads();
-        tail_restrict<D, CT>(Rs, lds + A.bo[l + 1], lds + A.xo[l + 1], A.g[l], A.g[l + 1]);
+        tail_restrict<D, CT>(Rs, BC, XC, g, gc);
         __syncthreads();
     };
     auto level_coarsest = [&](auto& P, int l) {
         CT* X = lds + A.xo[l];
         const CT* Bv = lds + A.bo[l];
-        const Geo& g = A.g[l];
+        const Geo g = A.g[l];
+        const int npass = A.npass;
         const int npairs = ((g.nx + 1) >> 1) * g.ny * (D == 3 ? g.nz : 1);
         if (npairs <= 64) {
             // every pair is owned by a lane of wave 0 (thread = pair): the
@@ -389,7 +396,7 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& Ag) {
             // of one wave complete in program order; the fences keep the
             // compiler from moving them across passes)
             if (tid < 64) {
-                for (int s = 0; s < A.npass; s++) {
+                for (int s = 0; s < npass; s++) {
                     tail_smooth<D>(P, X, Bv, g, (s & 1) ? 0 : 1);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -399,19 +406,21 @@ __device__ __forceinline__ void d_tail(const TailArgs<C4>& Ag) {
             __syncthreads();
             return;
         }
-        for (int s = 0; s < A.npass; s++) {     // odd, even, odd, ... (forward/reverse pairs collapsed)
+        for (int s = 0; s < npass; s++) {     // odd, even, odd, ... (forward/reverse pairs collapsed)
             tail_smooth<D>(P, X, Bv, g, (s & 1) ? 0 : 1);
             __syncthreads();
         }
     };
     auto level_up = [&](auto& P, int l) {
+        const Geo g = A.g[l], gc = A.g[l + 1];
         CT* X = lds + A.xo[l];
         const CT* Bv = lds + A.bo[l];
-        tail_prolong<D>(P, X, lds + A.xo[l + 1], A.g[l], A.g[l + 1]);
+        const CT* XC = lds + A.xo[l + 1];
+        tail_prolong<D>(P, X, XC, g, gc);
         __syncthreads();
-        tail_smooth<D>(P, X, Bv, A.g[l], 0);
+        tail_smooth<D>(P, X, Bv, g, 0);
         __syncthreads();
-        tail_smooth<D>(P, X, Bv, A.g[l], 1);
+        tail_smooth<D>(P, X, Bv, g, 1);
         __syncthreads();
     };
     // down
