@@ -964,8 +964,12 @@ public:
         graph_valid_ = false;
     }
     void run_chunk(T* x, const int* skip, int chunk, int every) {
-        join_x();   // an x update forked by an eager chunk precedes the graph's own
+        // Eager chunks need no join here: each iteration joins before the
+        // operator pass that overwrites the p buffer an unjoined x pass reads
+        // (the alternation continues across chunks).  A graph's own forks and
+        // joins assume nothing is pending when it starts.
         if (o_.use_graph && !timer_.on && chunk == every) {
+            join_x();   // an x update forked by an eager chunk precedes the graph's own
             const int k = pcur_;
             if (!graph_valid_ || graph_chunk_[k] != chunk) {
                 if (gexec_[k]) { (void)hipGraphExecDestroy(gexec_[k]); gexec_[k] = nullptr; }
