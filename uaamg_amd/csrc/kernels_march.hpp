@@ -226,7 +226,7 @@ __device__ __forceinline__ void d_down_m(const VB* __restrict__ b, FusedArgs<D, 
             }
             if constexpr (!L0) { S.ca[s][pos.e[u]] = cv.cx; if (D == 3) S.cb[s][pos.e[u]] = cv.cy; }
         }
-        issue(pfb, pfc, P + 2);
+        issue(pfb, pfc, min(P + 2, last + 2));   // past last + 2 (never used): the cached plane last + 2 again
         __syncthreads();
         if constexpr (XD) {   // odd cells of plane P: b/d with the formed diagonal
 #pragma unroll
@@ -407,7 +407,7 @@ __device__ __forceinline__ void d_up_m(const VB* __restrict__ b, FusedArgs<D, L0
             }
             if constexpr (!L0) { S.ca[s][pos.e[u]] = cv.cx; if (D == 3) S.cb[s][pos.e[u]] = cv.cy; }
         }
-        issue(pfb, pfc, pfx, P + 2);
+        issue(pfb, pfc, pfx, min(P + 2, last + 2));   // past last + 2 (never used): the cached plane last + 2 again
         __syncthreads();
         if constexpr (XD) {   // plane P: odd b/d with the formed diagonal, + 2 x_coarse on fluid cells
 #pragma unroll

@@ -152,7 +152,10 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                     PN[S][t][q] = (nb & NB_FLUID)
                         ? fmaf(beta, pfp[PB][t][q], pfz[PB][t][q] - mean) : 0.0f;
                 }
-            issue(PB, P + 2 * dir);
+            // the last plane to enter is last + 1 (upwards) or m0 - 1 (downwards);
+            // beyond it the loads repeat that cached plane (no memory traffic), so
+            // they stay unconditional and the waits stay per step
+            issue(PB, dir > 0 ? min(P + 2, th.last + 1) : max(P - 2, th.m0 - 1));
 #pragma unroll
             for (int q = 0; q < CPL; q++) {
                 L.ex[PB][w][0][q][lane] = PN[S][0][q];

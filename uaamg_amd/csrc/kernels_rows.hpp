@@ -354,7 +354,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
                 else X[S][t][q] = (ri != 0.0f) ? 0.0f : rv;
             });
         });
-        issue(PB, P + 2);
+        issue(PB, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again, no memory traffic
         sfor<CPL>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (((q + S) & 1) == 1) L.ex[EW][0][w][0][q / 2][lane] = X[S][0][q];
@@ -540,7 +540,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
                     else X[S][t][q] = w2;
                 });
             });
-            issue(PB, P + 2);
+            issue(PB, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again, no memory traffic
             sfor<CPL>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (((q + S) & 1) == 1) L.ex[EW][0][w][0][q / 2][lane] = X[S][0][q];
@@ -912,7 +912,7 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
                 else Q.X[S][t][q] = (ri != 0.0f) ? 0.0f : rv;
             });
         });
-        rec_issue<CPL, NW, RW, SV, C4>(pf[PB], b, rec, th, g, P + 2);
+        rec_issue<CPL, NW, RW, SV, C4>(pf[PB], b, rec, th, g, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again
         sfor<CPL>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
@@ -1052,7 +1052,7 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
                 else Q.X[S][t][q] = w2;
             });
         });
-        issue(PB, P + 2);
+        issue(PB, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again, no memory traffic
         sfor<CPL>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
