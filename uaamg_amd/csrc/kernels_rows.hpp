@@ -659,6 +659,12 @@ __device__ __forceinline__ void vals_load(const SV* __restrict__ p, int id, cons
     }
 }
 
+// planes the record kernels (levels 64 < nx <= 128) keep in flight: the
+// loads of a plane are issued REC_PD steps before it enters (a power of 2).
+// 4 measured slower at 128^3 (45 / 38 us instead of 19 / 21 us): the
+// extra buffers take the kernels past 168 VGPRs, into scratch.
+constexpr int REC_PD = 2;
+
 template <typename C4> struct RecE;
 template <> struct RecE<float4> { using E = float; };
 template <> struct RecE<BF16x4> { using E = __bf16; };
@@ -877,7 +883,7 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
     __syncthreads();
 
     RecQ<CPL, RW> Q;
-    RecPf<CPL, RW, C4> pf[2];
+    RecPf<CPL, RW, C4> pf[REC_PD];
     float acc[RW / 2][CPL / 2];
 #pragma unroll
     for (int s4 = 0; s4 < 4; s4++) {
@@ -894,13 +900,17 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
 #pragma unroll
         for (int c = 0; c < CPL / 2; c++) acc[a][c] = 0.0f;
     rec_prev_cz<CPL, NW, RW, C4>(Q, rec, th, g);
-    rec_issue<CPL, NW, RW, SV, C4>(pf[1], b, rec, th, g, th.m0 - 2);
-    rec_issue<CPL, NW, RW, SV, C4>(pf[0], b, rec, th, g, th.m0 - 1);
+    // planes m0 - 2 .. m0 + REC_PD - 3 in flight before the first step, each
+    // in the buffer of its slot (m0 = 1 mod 4: plane m0 - 2 + d has slot 3 + d)
+    sfor<REC_PD>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        rec_issue<CPL, NW, RW, SV, C4>(pf[(3 + d) & (REC_PD - 1)], b, rec, th, g, th.m0 - 2 + d);
+    });
 
     auto step = [&](auto Sc, int P) {
         constexpr int S = decltype(Sc)::value;
         constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;
-        constexpr int PB = S & 1, EW = S & 1, ER = EW ^ 1;
+        constexpr int PB = S & (REC_PD - 1), EW = S & 1, ER = EW ^ 1;
         // S1: plane P enters; odd cells x0 = b / d; isolated cells carry b
         rec_enter<S, CPL, NW, RW, C4>(Q, pf[PB], th, P >= 1 && P <= g.nz, L);
         sfor<RW>([&](auto tc) {
@@ -912,7 +922,7 @@ __device__ __forceinline__ void d_rows_downc(const SV* __restrict__ b, const C4*
                 else Q.X[S][t][q] = (ri != 0.0f) ? 0.0f : rv;
             });
         });
-        rec_issue<CPL, NW, RW, SV, C4>(pf[PB], b, rec, th, g, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again
+        rec_issue<CPL, NW, RW, SV, C4>(pf[PB], b, rec, th, g, min(P + REC_PD, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again
         sfor<CPL>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
@@ -999,9 +1009,9 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
     __syncthreads();
 
     RecQ<CPL, RW> Q;
-    RecPf<CPL, RW, C4> pf[2];
-    float pfx[2][RW / 2][CPL / 2];
-    bool pcin[2];
+    RecPf<CPL, RW, C4> pf[REC_PD];
+    float pfx[REC_PD][RW / 2][CPL / 2];
+    bool pcin[REC_PD];
     int coff[RW / 2][CPL / 2];
     bool cok[RW / 2][CPL / 2];
 #pragma unroll
@@ -1033,13 +1043,17 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
             for (int c = 0; c < CPL / 2; c++) pfx[slot][a][c] = (float)xc[coff[a][c] + cpo];
     };
     rec_prev_cz<CPL, NW, RW, C4>(Q, rec, th, g);
-    issue(1, th.m0 - 2);
-    issue(0, th.m0 - 1);
+    // planes m0 - 2 .. m0 + REC_PD - 3 in flight before the first step (plane
+    // m0 - 2 + d has slot 3 + d)
+    sfor<REC_PD>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        issue((3 + d) & (REC_PD - 1), th.m0 - 2 + d);
+    });
 
     auto step = [&](auto Sc, int P) {
         constexpr int S = decltype(Sc)::value;
         constexpr int S1 = (S + 3) & 3, S2 = (S + 2) & 3, S3 = (S + 1) & 3;
-        constexpr int PB = S & 1, EW = S & 1, ER = EW ^ 1;
+        constexpr int PB = S & (REC_PD - 1), EW = S & 1, ER = EW ^ 1;
         // S1: plane P: x' = x0 + 2 x_coarse on active cells
         rec_enter<S, CPL, NW, RW, C4>(Q, pf[PB], th, P >= 1 && P <= g.nz, L);
         sfor<RW>([&](auto tc) {
@@ -1052,7 +1066,7 @@ __device__ __forceinline__ void d_rows_upc(const SV* __restrict__ b, const C4* _
                 else Q.X[S][t][q] = w2;
             });
         });
-        issue(PB, min(P + 2, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again, no memory traffic
+        issue(PB, min(P + REC_PD, th.last + 2));   // past last + 2 (never used): the cached plane last + 2 again, no memory traffic
         sfor<CPL>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (((q + S) & 1) == 1) L.r.ex[EW][0][w][0][q / 2][lane] = Q.X[S][0][q];
