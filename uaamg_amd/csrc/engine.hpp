@@ -670,6 +670,9 @@ public:
                 F.b, F.coef, F.g, rg, C.x, C.g, F.x, skip);
         }
     }
+    // every lane's cells lie inside the row (nx == 64 cells-per-lane): the
+    // row kernels' instantiations without lane masks
+    bool rows_full() const { return rows_cpl_ > 0 && g0_.nx == 64 * rows_cpl_; }
     void rows_down0(const T* r, Level& C, const int* skip) {
         if constexpr (D == 3 && std::is_same<T, float>::value) {
             if (fr_old_) {   // alpha from the operator's partials; r = r_old - alpha Ap written to r; r.r -> convergence
@@ -677,7 +680,9 @@ public:
                 // UAAMG_FORM_NT_D plain): 34.5 vs 33.5 solves/s at 256^3 in three paired runs
                 const bool dnt = (o_.forms & UAAMG_FORM_NT_D) && rows_cpl_ == 4;
                 forms_used_ |= dnt ? UAAMG_FORM_NT_D : 0;
-                tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? (dnt ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S, true> : k_rows_down0r<4, ROWS_NW, ROWS_RW, S>)
+                tlaunch(timer_, UAAMG_K_DOWN0, rows_cpl_ == 4 ? (dnt ? (rows_full() ? k_rows_down0r<4, ROWS_NW, ROWS_RW, S, true, true>
+                                                                                    : k_rows_down0r<4, ROWS_NW, ROWS_RW, S, true>)
+                                                                     : k_rows_down0r<4, ROWS_NW, ROWS_RW, S>)
                                                               : k_rows_down0r<2, ROWS_NW, ROWS_RW, S>,
                         dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, (const float*)fr_old_, (const uint8_t*)nbr0_, g0_, rtab_,
                         rgeo_, C.b, C.g, skip, (const float*)Ap_, (float*)r, partials_, (const double*)partials_m_,
@@ -696,7 +701,9 @@ public:
             // 33.9 solves/s at 256^3 in three paired runs
             const bool unt = (o_.forms & UAAMG_FORM_NT_U) && rows_cpl_ == 4;
             forms_used_ |= unt ? UAAMG_FORM_NT_U : 0;
-            tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? (unt ? k_rows_up0<4, ROWS_NW, ROWS_RW, S, true> : k_rows_up0<4, ROWS_NW, ROWS_RW, S>)
+            tlaunch(timer_, UAAMG_K_UP0, rows_cpl_ == 4 ? (unt ? (rows_full() ? k_rows_up0<4, ROWS_NW, ROWS_RW, S, true, true>
+                                                                              : k_rows_up0<4, ROWS_NW, ROWS_RW, S, true>)
+                                                               : k_rows_up0<4, ROWS_NW, ROWS_RW, S>)
                                                         : k_rows_up0<2, ROWS_NW, ROWS_RW, S>,
                     dim3(rows_blocks_), dim3(ROWS_NW * 64), stream_, r, nbr0_, g0_, rtab_, rgeo_, C.x, C.g, z, skip, partials_);
             // with the split update the next operator pass does this finish
@@ -1021,7 +1028,9 @@ public:
                 // UAAMG_FORM_NT_P plain): 33.6 vs 33.0 solves/s at 256^3 in three paired runs
                 const bool pnt = (o_.forms & UAAMG_FORM_NT_P) && fr_ && rows_cpl_ == 4;
                 forms_used_ |= pnt ? UAAMG_FORM_NT_P : 0;
-                auto kern = fr_ ? (rows_cpl_ == 4 ? (pnt ? k_rows_pmv<4, PMV_NW, PMV_RW, true, true> : k_rows_pmv<4, PMV_NW, PMV_RW, true>)
+                auto kern = fr_ ? (rows_cpl_ == 4 ? (pnt ? (rows_full() ? k_rows_pmv<4, PMV_NW, PMV_RW, true, true, true>
+                                                                        : k_rows_pmv<4, PMV_NW, PMV_RW, true, true>)
+                                                         : k_rows_pmv<4, PMV_NW, PMV_RW, true>)
                                                   : k_rows_pmv<2, PMV_NW, PMV_RW, true>)
                                 : (rows_cpl_ == 4 ? k_rows_pmv<4, PMV_NW, PMV_RW, false> : k_rows_pmv<2, PMV_NW, PMV_RW, false>);
                 tlaunch(timer_, UAAMG_K_MATVEC, kern, dim3(pmv_blocks_), dim3(PMV_NW * 64), stream_, (const float*)z_,

@@ -51,7 +51,7 @@ struct PmvLDS {
 // and beta; workgroup 0 stores them.  rsold and the first-iteration flag are
 // read here and only written by the next kernel (the level-0 down kernel),
 // so no workgroup can see a value of this launch.
-template <int CPL, int NW, int RW, bool FZ = false, bool NT = false>
+template <int CPL, int NW, int RW, bool FZ = false, bool NT = false, bool FULL = false>
 __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                            float* __restrict__ pnew, float* __restrict__ Ap,
                                            const uint8_t* __restrict__ nbr, const Geo& g, const PmvTab& K,
@@ -69,7 +69,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     // first or both last), and the second read can hit the L2: at 256^3 the
     // pass fetches 195.9 instead of 199.3 MB (28.44 vs 28.54 ms per solve)
     const int dir = (kc & 1) ? -1 : 1;
-    RowThread<CPL, NW, RW, 1> th;
+    RowThread<CPL, NW, RW, 1, FULL> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
 
@@ -164,6 +164,29 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
             const int R = P - dir;
             if (R >= th.m0 && R <= th.last) {
                 constexpr int RB = S1 & 1;   // LDS buffer written when plane R entered
+                // the diagonals of the plane's cells: every table read before
+                // the first use, the Dirichlet excess in one uniform branch
+                double dP[RW][CPL];
+#pragma unroll
+                for (int t = 0; t < RW; t++)
+#pragma unroll
+                    for (int q = 0; q < CPL; q++) dP[t][q] = sd[NB[S1][t][q] >> 1];
+                if (K.any_dirichlet) {
+#pragma unroll
+                    for (int t = 0; t < RW; t++)
+#pragma unroll
+                        for (int q = 0; q < CPL; q++) {
+                            const int i = th.i0 + q, j = th.jrow[t];
+                            double d = dP[t][q];
+                            if (i == 1) d += K.ex[0];
+                            if (i == g.nx) d += K.ex[1];
+                            if (j == 1) d += K.ex[2];
+                            if (j == g.ny) d += K.ex[3];
+                            if (R == 1) d += K.ex[4];
+                            if (R == g.nz) d += K.ex[5];
+                            dP[t][q] = d;
+                        }
+                }
                 sfor<RW>([&](auto tc) {
                     constexpr int t = decltype(tc)::value;
                     const float el = lane_from_left(PN[S1][t][CPL - 1]);
@@ -179,16 +202,7 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
                         if constexpr (t > 0) vb = PN[S1][t - 1][q]; else vb = L.ex[RB][w > 0 ? w - 1 : 0][1][q][lane];
                         if constexpr (t < RW - 1) vt = PN[S1][t + 1][q]; else vt = L.ex[RB][w < NW - 1 ? w + 1 : NW - 1][0][q][lane];
                         const uint8_t nb = NB[S1][t][q];
-                        double d = sd[nb >> 1];
-                        if (K.any_dirichlet) {
-                            const int i = th.i0 + q, j = th.jrow[t];
-                            if (i == 1) d += K.ex[0];
-                            if (i == g.nx) d += K.ex[1];
-                            if (j == 1) d += K.ex[2];
-                            if (j == g.ny) d += K.ex[3];
-                            if (R == 1) d += K.ex[4];
-                            if (R == g.nz) d += K.ex[5];
-                        }
+                        const double d = dP[t][q];
                         const double sum = K.idx2 * ((double)vl + (double)vr) + K.idy2 * ((double)vb + (double)vt)
                                          + K.idz2 * ((double)PN[S2][t][q] + (double)PN[S][t][q]);
                         const bool fl = nb & NB_FLUID;
@@ -228,13 +242,13 @@ __device__ __forceinline__ void d_rows_pmv(const float* __restrict__ z, const fl
     block_store<1>(acc, partials);
 }
 
-template <int CPL, int NW, int RW, bool FZ = false, bool NT = false>
+template <int CPL, int NW, int RW, bool FZ = false, bool NT = false, bool FULL = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_pmv(const float* __restrict__ z, const float* __restrict__ pold,
                                                       float* __restrict__ pnew, float* __restrict__ Ap,
                                                       const uint8_t* __restrict__ nbr, Geo g, PmvTab K,
                                                       RowGeo rg, double* partials, Ctrl* ctrl,
                                                       const double* zp, int nzp) {
-    d_rows_pmv<CPL, NW, RW, FZ, NT>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x),
+    d_rows_pmv<CPL, NW, RW, FZ, NT, FULL>(z, pold, pnew, Ap, nbr, g, K, rg, partials, ctrl, row_tile_of_block(rg, blockIdx.x),
                                 zp, nzp);
 }
 

@@ -70,22 +70,45 @@ __device__ __forceinline__ int row_tile_of_block(const RowGeo& rg, int b) {
     return (b & 7) * rg.per_xcd + (b >> 3);
 }
 
-__device__ __forceinline__ float rows_rinv(uint8_t nb, const float* srinv, const float* sd, const RowTab& T,
-                                           const Geo& g, int i, int j, int k) {
-    const bool fl = nb & NB_FLUID;
-    if (!T.any_dirichlet) {   // uniform branch; the table read itself is unconditional
-        const float ri = srinv[nb >> 1];
-        return fl ? ri : 0.0f;
+// 1/d of the RW x CPL cells of a lane's plane (0 on non-fluid cells and
+// where d < 1e-30): without Dirichlet walls the table value; with them d =
+// the table's d plus each touching wall's excess, divided in FP32.  Every
+// table read is issued before the first is used (one LDS round trip per
+// plane, not one per cell) and the Dirichlet test is one uniform branch.
+template <int RW, int CPL>
+__device__ __forceinline__ void rows_rinv_plane(const uint8_t (&nb)[RW][CPL], float (&ri)[RW][CPL], const float* srinv,
+                                                const float* sd, const RowTab& T, const Geo& g, int i0,
+                                                const int (&jrow)[RW], int k) {
+    float v[RW][CPL];
+    if (!T.any_dirichlet) {
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) v[t][q] = srinv[nb[t][q] >> 1];
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) ri[t][q] = (nb[t][q] & NB_FLUID) ? v[t][q] : 0.0f;
+    } else {
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) v[t][q] = sd[nb[t][q] >> 1];
+#pragma unroll
+        for (int t = 0; t < RW; t++)
+#pragma unroll
+            for (int q = 0; q < CPL; q++) {
+                float d = v[t][q];
+                const int i = i0 + q, j = jrow[t];
+                if (i == 1) d += T.ex[0];
+                if (i == g.nx) d += T.ex[1];
+                if (j == 1) d += T.ex[2];
+                if (j == g.ny) d += T.ex[3];
+                if (k == 1) d += T.ex[4];
+                if (k == g.nz) d += T.ex[5];
+                ri[t][q] = (nb[t][q] & NB_FLUID) ? (d >= 1e-30f ? 1.0f / d : 0.0f) : 0.0f;
+            }
     }
-    if (!fl) return 0.0f;
-    float d = sd[nb >> 1];
-    if (i == 1) d += T.ex[0];
-    if (i == g.nx) d += T.ex[1];
-    if (j == 1) d += T.ex[2];
-    if (j == g.ny) d += T.ex[3];
-    if (k == 1) d += T.ex[4];
-    if (k == g.nz) d += T.ex[5];
-    return d >= 1e-30f ? 1.0f / d : 0.0f;
 }
 
 // The CPL values and neighbour bytes of one lane's cells in one vector load
@@ -151,8 +174,11 @@ __device__ __forceinline__ void sfor(F&& f) {
 // Shared set-up of the row kernels (H halo rows on each side of the tile).
 // Addresses are 32-bit element offsets: a row's cells are roff[t] + q.  Rows
 // and planes outside the grid are clamped to the ghost layer (neighbour byte
-// 0: read as non-fluid), lanes past nx are masked with okx.
-template <int CPL, int NW, int RW, int H = 2>
+// 0: read as non-fluid), lanes past nx are masked with okx.  FULL: the
+// launch guarantees nx == 64 CPL, so okx and full are compile-time true and
+// every lane mask folds away (the masks cost about a tenth of the level-0
+// kernels' VALU instructions).
+template <int CPL, int NW, int RW, int H = 2, bool FULL = false>
 struct RowThread {
     static constexpr int TY = NW * RW - 2 * H;
     int lane, w, i0, b0, m0, last;
@@ -166,13 +192,13 @@ struct RowThread {
         // the wave index is uniform: in a scalar register, the row tests
         // below (jrow, tile_row) are scalar branches, not exec-mask ones
         w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-        full = g.nx == 64 * CPL;
+        full = FULL || g.nx == 64 * CPL;
         i0 = 1 + CPL * lane;
         b0 = 1 + ty * TY;
         m0 = 1 + kc * rg.KC;
         last = min(m0 + rg.KC - 1, g.nz);
 #pragma unroll
-        for (int q = 0; q < CPL; q++) okx[q] = i0 + q <= g.nx;
+        for (int q = 0; q < CPL; q++) okx[q] = FULL || i0 + q <= g.nx;
 #pragma unroll
         for (int t = 0; t < RW; t++) {
             jrow[t] = b0 - H + w * RW + t;
@@ -219,7 +245,7 @@ struct RUpd {
     int xslot = 0;                   // ctrl->xa slot of this iteration's x pass (its p buffer)
 };
 
-template <int CPL, int NW, int RW, typename VC, bool FR = false, bool NT = false>
+template <int CPL, int NW, int RW, typename VC, bool FR = false, bool NT = false, bool FULL = false>
 __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
                                              const RowTab& K, const RowGeo& rg, VC* __restrict__ bc, const Geo& cg,
                                              const int* skip, int tile, const RUpd& u = RUpd{}) {
@@ -260,7 +286,7 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
     }
     const int ty = tile % rg.nty, kc = tile / rg.nty;
     double rr[1] = {0.0};
-    RowThread<CPL, NW, RW> th;
+    RowThread<CPL, NW, RW, 2, FULL> th;
     th.init(g, rg, ty, kc);
     const int lane = th.lane, w = th.w;
     for (int e = threadIdx.x; e < 64; e += NW * 64) { L.tab[0][e] = K.rinv[e]; L.tab[1][e] = K.d[e]; }
@@ -339,13 +365,15 @@ __device__ __forceinline__ void d_rows_down0(const float* __restrict__ r, const 
                 }
             });
         }
+        float riP[RW][CPL];
+        rows_rinv_plane<RW, CPL>(pfn[PB], riP, L.tab[0], L.tab[1], K, g, th.i0, th.jrow, P);
         sfor<RW>([&](auto tc) {
             constexpr int t = decltype(tc)::value;
             sfor<CPL>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 const uint8_t nb = pfn[PB][t][q];
                 const float rv = (nb & NB_FLUID) ? pfr[PB][t][q] : 0.0f;
-                const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
+                const float ri = riP[t][q];
                 Rq[S & 1][t][q] = rv;
                 Iq[S][t][q] = ri;
                 // isolated fluid cells (ri == 0, no fluid neighbour: their x is
@@ -447,7 +475,7 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0(const float* __restrict_
 
 // down0 with the fused residual update (r_old -> r_new, r.r partials) and
 // the operator's finish (alpha from the p.Ap partials)
-template <int CPL, int NW, int RW, typename VC, bool NT = false>
+template <int CPL, int NW, int RW, typename VC, bool NT = false, bool FULL = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict__ rold, const uint8_t* __restrict__ nbr,
                                                          Geo g, RowTab K, RowGeo rg, VC* __restrict__ bc, Geo cg,
                                                          const int* skip, const float* __restrict__ Ap,
@@ -455,13 +483,13 @@ __global__ __launch_bounds__(NW * 64) void k_rows_down0r(const float* __restrict
                                                          const double* pap, int npap, Ctrl* ctrl, int xslot) {
     RUpd u;
     u.Ap = Ap; u.rnew = rnew; u.partials = partials; u.pap = pap; u.npap = npap; u.ctrl = ctrl; u.xslot = xslot;
-    d_rows_down0<CPL, NW, RW, VC, true, NT>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
+    d_rows_down0<CPL, NW, RW, VC, true, NT, FULL>(rold, nbr, g, K, rg, bc, cg, skip, row_tile_of_block(rg, blockIdx.x), u);
 }
 
 // ------------------------------------------------------------------ up
 // r, coarse x -> z (prolongation + reverse sweep), with sum z, sum r z, sum r
 // over fluid cells reduced per workgroup into partials.
-template <int CPL, int NW, int RW, typename VCX, bool NT = false>
+template <int CPL, int NW, int RW, typename VCX, bool NT = false, bool FULL = false>
 __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr, const Geo& g,
                                            const RowTab& K, const RowGeo& rg, const VCX* __restrict__ xc, const Geo& cg,
                                            float* __restrict__ z, const int* skip, double* partials, int tile) {
@@ -470,7 +498,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
     __shared__ RowLDS<CPL, NW, RW> L;
     const bool active = tile < rg.ntiles;   // idle workgroups still store zero partials
     const int ty = tile % rg.nty, kc = tile / rg.nty;
-    RowThread<CPL, NW, RW> th;
+    RowThread<CPL, NW, RW, 2, FULL> th;
     th.init(g, rg, active ? ty : 0, active ? kc : 0);
     const int lane = th.lane, w = th.w;
     for (int e = threadIdx.x; e < 64; e += NW * 64) { L.tab[0][e] = K.rinv[e]; L.tab[1][e] = K.d[e]; }
@@ -525,6 +553,8 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
             // S1: plane P: x' = x0 + 2 x_coarse (x0 = r/d on odd cells; the
             // even x0 is replaced by the reverse sweep, except on isolated cells
             // where it is 0)
+            float riP[RW][CPL];
+            rows_rinv_plane<RW, CPL>(pfn[PB], riP, L.tab[0], L.tab[1], K, g, th.i0, th.jrow, P);
             sfor<RW>([&](auto tc) {
                 constexpr int t = decltype(tc)::value;
                 sfor<CPL>([&](auto qc) {
@@ -532,7 +562,7 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
                     const uint8_t nb = pfn[PB][t][q];
                     const bool fl = nb & NB_FLUID;
                     const float rv = fl ? pfr[PB][t][q] : 0.0f;
-                    const float ri = rows_rinv(nb, L.tab[0], L.tab[1], K, g, th.i0 + q, th.jrow[t], P);
+                    const float ri = riP[t][q];
                     const float w2 = (fl && pcin[PB] && cok[t / 2][q / 2]) ? 2.0f * pfx[PB][t / 2][q / 2] : 0.0f;
                     Rq[S][t][q] = rv;
                     Iq[S][t][q] = ri;
@@ -625,11 +655,11 @@ __device__ __forceinline__ void d_rows_up0(const float* __restrict__ r, const ui
     block_store<3>(red, partials);
 }
 
-template <int CPL, int NW, int RW, typename VCX, bool NT = false>
+template <int CPL, int NW, int RW, typename VCX, bool NT = false, bool FULL = false>
 __global__ __launch_bounds__(NW * 64) void k_rows_up0(const float* __restrict__ r, const uint8_t* __restrict__ nbr,
                                                       Geo g, RowTab K, RowGeo rg, const VCX* __restrict__ xc, Geo cg,
                                                       float* __restrict__ z, const int* skip, double* partials) {
-    d_rows_up0<CPL, NW, RW, VCX, NT>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
+    d_rows_up0<CPL, NW, RW, VCX, NT, FULL>(r, nbr, g, K, rg, xc, cg, z, skip, partials, row_tile_of_block(rg, blockIdx.x));
 }
 
 // ================================================================== record levels
