@@ -94,6 +94,13 @@ def parity_cases():
                     wall=[ob.NEUMANN, ob.DIRICHLET, ob.NEUMANN, ob.NEUMANN, ob.DIRICHLET, ob.NEUMANN])
     pb.solid = random_solids(pb, rng, frac=0.2, nobj=10)
     out.append(("3d_200x70x40_walls_solids", pb))
+    # rows of exactly 256 cells (4 per lane, every lane inside the row): the
+    # level-0 kernels without lane masks, with the Dirichlet excess on the
+    # first and the last lane's cells
+    pb = ob.Problem((256, 20, 12), (1.0, 0.1, 0.05),
+                    wall=[ob.DIRICHLET, ob.DIRICHLET, ob.DIRICHLET, ob.NEUMANN, ob.NEUMANN, ob.DIRICHLET])
+    pb.solid = balls(pb, [(0.3, 0.05, 0.025), (0.7, 0.04, 0.02)], [0.03, 0.02])
+    out.append(("3d_256x20x12_walls", pb))
     # porous: 40 % random solid cells, many fluid cells with no fluid neighbour
     pb = ob.Problem((20, 18, 16))
     m = np.zeros(pb.shape[::-1], np.uint8)
